@@ -1,0 +1,9 @@
+"""fallingsand3d_amd -- MI355X-native (gfx950) falling-sand voxel tick.
+
+The compute path is libfs3d.so (hand-written HIP kernels behind the C ABI in
+include/fs3d.h); this package is its Python host mirror.  There is no CPU
+fallback: without the built library or a GPU every compute call raises Fs3dError.
+"""
+from ._lib import Fs3dError, LIB_PATH, lib  # noqa: F401
+from .world import (EMPTY, FLUID, SAND, STATIC, BOXES_512, PILLARS_2048, World,  # noqa: F401
+                    batch_specs, device_count, init, rule_lut, serve_specs, threshold)

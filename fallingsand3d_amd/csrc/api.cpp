@@ -1,0 +1,473 @@
+// C ABI of the fs3d library: errors, device context, world grid, tick, hash,
+// captured multi-tick runs and scene fills.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fs3d_internal.h"
+
+namespace fs3d {
+
+namespace {
+thread_local std::string g_error;
+
+struct DeviceCtx {
+  bool ready = false;
+  u64 *rule = nullptr;  // 4 x 65536 x 8 bytes
+};
+constexpr int kMaxDevices = 16;
+DeviceCtx g_dev[kMaxDevices];
+std::mutex g_dev_mutex;
+}  // namespace
+
+int set_error(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int clear_error() {
+  g_error.clear();
+  return FS3D_OK;
+}
+
+int ensure_device(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return set_error(FS3D_ERR_INVALID, "device %d out of range", dev);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return set_error(FS3D_ERR_NODEVICE, "no HIP device is visible");
+  if (dev >= count) return set_error(FS3D_ERR_NODEVICE, "device %d not present (%d visible)", dev, count);
+  FS3D_HIP(hipSetDevice(dev));
+  std::lock_guard<std::mutex> lock(g_dev_mutex);
+  DeviceCtx &c = g_dev[dev];
+  if (c.ready) return FS3D_OK;
+  hipDeviceProp_t prop;
+  FS3D_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(FS3D_ERR_NODEVICE, "device %d is %s; this library is built for gfx950 only", dev,
+                     prop.gcnArchName);
+  std::vector<u64> host((size_t)4 * 65536);
+  build_rule_entries(host.data());
+  FS3D_HIP(hipMalloc(&c.rule, host.size() * sizeof(u64)));
+  FS3D_HIP(hipMemcpy(c.rule, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice));
+  c.ready = true;
+  return FS3D_OK;
+}
+
+const u64 *device_rule_table(int dev) { return g_dev[dev].rule; }
+
+}  // namespace fs3d
+
+using namespace fs3d;
+
+namespace {
+
+int check_world(const fs3d_world *w) {
+  if (!w) return set_error(FS3D_ERR_INVALID, "world handle is NULL");
+  return FS3D_OK;
+}
+
+int ensure_small(fs3d_world *w, int slots) {
+  // d_small[0] = device tick counter, d_small[1..slots] = hash outputs
+  if (slots <= w->small_cap) return FS3D_OK;
+  int cap = std::max(slots, 64);
+  u64 *d = nullptr;
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  FS3D_HIP(hipMalloc(&d, (size_t)(cap + 1) * sizeof(u64)));
+  FS3D_HIP(hipMemset(d, 0, (size_t)(cap + 1) * sizeof(u64)));
+  if (w->d_small) FS3D_HIP(hipFree(w->d_small));
+  w->d_small = d;
+  w->small_cap = cap;
+  // cached graphs captured the old pointers
+  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+  w->graphs.clear();
+  if (w->h_pinned_cap < cap + 1) {
+    if (w->h_pinned) FS3D_HIP(hipHostFree(w->h_pinned));
+    w->h_pinned = nullptr;
+    FS3D_HIP(hipHostMalloc(&w->h_pinned, (size_t)(cap + 1) * sizeof(u64), hipHostMallocDefault));
+    w->h_pinned_cap = cap + 1;
+  }
+  return FS3D_OK;
+}
+
+void release(fs3d_world *w) {
+  (void)hipSetDevice(w->dev);
+  if (w->stream) (void)hipStreamSynchronize(w->stream);
+  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+  w->graphs.clear();
+  for (int i = 0; i < 2; i++)
+    if (w->buf[i]) (void)hipFree(w->buf[i]);
+  if (w->d_parts) (void)hipFree(w->d_parts);
+  if (w->d_small) (void)hipFree(w->d_small);
+  if (w->h_pinned) (void)hipHostFree(w->h_pinned);
+  if (w->ev0) (void)hipEventDestroy(w->ev0);
+  if (w->ev1) (void)hipEventDestroy(w->ev1);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  delete w;
+}
+
+int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev) {
+  const u64 *rule = device_rule_table(w->dev);
+  for (int k = 0; k < nticks; k++) {
+    const bool hashed = hash_every > 0 && (k + 1) % hash_every == 0;
+    const int src = (w->cur + k) & 1;
+    FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                         (u32)(w->tick + (u64)k), tick_dev, (u32)k, rule,
+                         hashed ? w->d_parts : nullptr, w->stream));
+    if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
+  }
+  return FS3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fs3d_last_error(void) { return g_error.c_str(); }
+
+int fs3d_abi_version(void) { return FS3D_ABI_VERSION; }
+
+int fs3d_device_count(int *count) {
+  if (!count) return set_error(FS3D_ERR_INVALID, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return clear_error();
+}
+
+int fs3d_init(int device) {
+  int rc = ensure_device(device);
+  return rc ? rc : clear_error();
+}
+
+int fs3d_rule_lut(uint16_t *lut) {
+  if (!lut) return set_error(FS3D_ERR_INVALID, "lut is NULL");
+  for (int v = 0; v < 4; v++)
+    for (u32 s = 0; s < 65536u; s++) lut[(size_t)v * 65536u + s] = rule_state_packed((u16)s, v);
+  return clear_error();
+}
+
+int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
+  (void)flags;
+  if (!dims || !out) return set_error(FS3D_ERR_INVALID, "dims/out is NULL");
+  *out = nullptr;
+  for (int a = 0; a < 3; a++)
+    if (dims[a] < 1) return set_error(FS3D_ERR_INVALID, "dims must be >= 1 (got %d,%d,%d)", dims[0], dims[1], dims[2]);
+  if (dims[0] > kMaxNx) return set_error(FS3D_ERR_INVALID, "nx=%d exceeds %d", dims[0], kMaxNx);
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  if (n > (1ull << 40)) return set_error(FS3D_ERR_INVALID, "grid too large");
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int rc = ensure_device(dev);
+  if (rc) return rc;
+  fs3d_world *w = new fs3d_world();
+  w->dev = dev;
+  w->nx = dims[0];
+  w->ny = dims[1];
+  w->nz = dims[2];
+  w->n = n;
+  w->alloc = (n + kGuardBytes + 255) & ~(size_t)255;
+  rc = plan_tick(w->nx, w->ny, w->nz, 0, &w->shape);
+  if (rc) {
+    delete w;
+    return rc;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->buf[i], w->alloc);
+  if (e == hipSuccess) e = hipMalloc(&w->d_parts, (size_t)kHashParts * kHashStride * sizeof(u64));
+  if (e == hipSuccess) e = hipEventCreate(&w->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&w->ev1);
+  if (e != hipSuccess) {
+    const bool oom = e == hipErrorOutOfMemory;
+    release(w);
+    (void)hipGetLastError();
+    return set_error(oom ? FS3D_ERR_NOMEM : FS3D_ERR_HIP, "world allocation (%zu bytes x2) failed: %s",
+                     (size_t)n, hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(w->d_parts, 0, (size_t)kHashParts * kHashStride * sizeof(u64), w->stream);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMemsetAsync(w->buf[i], 0, w->alloc, w->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  if (e != hipSuccess) {
+    release(w);
+    return set_error(FS3D_ERR_HIP, "world init failed: %s", hipGetErrorString(e));
+  }
+  rc = ensure_small(w, 64);
+  if (rc) {
+    release(w);
+    return rc;
+  }
+  *out = w;
+  return clear_error();
+}
+
+int fs3d_world_destroy(fs3d_world *w) {
+  if (w) release(w);
+  return clear_error();
+}
+
+int fs3d_world_dims(const fs3d_world *w, int32_t dims[3]) {
+  if (int rc = check_world(w)) return rc;
+  if (!dims) return set_error(FS3D_ERR_INVALID, "dims is NULL");
+  dims[0] = w->nx;
+  dims[1] = w->ny;
+  dims[2] = w->nz;
+  return clear_error();
+}
+
+static int voxel_offset(const fs3d_world *w, int32_t x, int32_t y, int32_t z, size_t *off) {
+  if (x < 0 || y < 0 || z < 0 || x >= w->nx || y >= w->ny || z >= w->nz)
+    return set_error(FS3D_ERR_INVALID, "voxel (%d,%d,%d) outside %dx%dx%d", x, y, z, w->nx, w->ny, w->nz);
+  *off = ((size_t)z * w->ny + y) * w->nx + x;
+  return FS3D_OK;
+}
+
+int fs3d_world_set_voxel(fs3d_world *w, int32_t x, int32_t y, int32_t z, uint8_t v) {
+  if (int rc = check_world(w)) return rc;
+  size_t off;
+  if (int rc = voxel_offset(w, x, y, z, &off)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipMemcpyAsync(w->buf[w->cur] + off, &v, 1, hipMemcpyHostToDevice, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_get_voxel(fs3d_world *w, int32_t x, int32_t y, int32_t z, uint8_t *v) {
+  if (int rc = check_world(w)) return rc;
+  if (!v) return set_error(FS3D_ERR_INVALID, "v is NULL");
+  size_t off;
+  if (int rc = voxel_offset(w, x, y, z, &off)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipMemcpyAsync(v, w->buf[w->cur] + off, 1, hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_upload(fs3d_world *w, const uint8_t *host, size_t nbytes) {
+  if (int rc = check_world(w)) return rc;
+  if (!host || nbytes != w->n) return set_error(FS3D_ERR_INVALID, "upload needs %zu bytes (got %zu)", w->n, nbytes);
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_download(fs3d_world *w, uint8_t *host, size_t nbytes) {
+  if (int rc = check_world(w)) return rc;
+  if (!host || nbytes != w->n) return set_error(FS3D_ERR_INVALID, "download needs %zu bytes (got %zu)", w->n, nbytes);
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipMemcpyAsync(host, w->buf[w->cur], w->n, hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_get_tick(const fs3d_world *w, uint64_t *tick) {
+  if (int rc = check_world(w)) return rc;
+  if (!tick) return set_error(FS3D_ERR_INVALID, "tick is NULL");
+  *tick = w->tick;
+  return clear_error();
+}
+
+int fs3d_world_set_tick(fs3d_world *w, uint64_t tick) {
+  if (int rc = check_world(w)) return rc;
+  w->tick = tick;
+  return clear_error();
+}
+
+int fs3d_world_set_segment(fs3d_world *w, int32_t seg) {
+  if (int rc = check_world(w)) return rc;
+  TickShape sh;
+  if (int rc = plan_tick(w->nx, w->ny, w->nz, seg, &sh)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  w->shape = sh;
+  w->seg_req = seg;
+  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+  w->graphs.clear();
+  return clear_error();
+}
+
+int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg) {
+  if (int rc = check_world(w)) return rc;
+  if (!seg) return set_error(FS3D_ERR_INVALID, "seg is NULL");
+  *seg = w->shape.seg;
+  return clear_error();
+}
+
+int fs3d_world_step(fs3d_world *w, int32_t nticks) {
+  if (int rc = check_world(w)) return rc;
+  if (nticks < 0) return set_error(FS3D_ERR_INVALID, "nticks < 0");
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (int rc = enqueue_ticks(w, nticks, 0, nullptr)) return rc;
+  w->cur = (w->cur + nticks) & 1;
+  w->tick += (u64)nticks;
+  return clear_error();
+}
+
+int fs3d_world_tick(fs3d_world *w, uint64_t *hash) {
+  if (int rc = check_world(w)) return rc;
+  if (!hash) return set_error(FS3D_ERR_INVALID, "hash is NULL");
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (int rc = enqueue_ticks(w, 1, 1, nullptr)) return rc;
+  FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  w->cur ^= 1;
+  w->tick += 1;
+  *hash = w->h_pinned[0];
+  return clear_error();
+}
+
+int fs3d_world_hash(fs3d_world *w, uint64_t *hash) {
+  if (int rc = check_world(w)) return rc;
+  if (!hash) return set_error(FS3D_ERR_INVALID, "hash is NULL");
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(launch_hash(w->buf[w->cur], w->n, w->d_parts, w->stream));
+  FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + 1, w->stream));
+  FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  *hash = w->h_pinned[0];
+  return clear_error();
+}
+
+int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *hashes) {
+  if (int rc = check_world(w)) return rc;
+  if (nticks < 0 || hash_every < 0) return set_error(FS3D_ERR_INVALID, "nticks/hash_every < 0");
+  const int nh = hash_every > 0 ? nticks / hash_every : 0;
+  if (nh > 0 && !hashes) return set_error(FS3D_ERR_INVALID, "hashes is NULL");
+  if (nticks == 0) return clear_error();
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (int rc = ensure_small(w, nh + 1)) return rc;
+  const auto key = std::make_tuple((int)nticks, (int)hash_every, w->cur);
+  auto it = w->graphs.find(key);
+  if (it == w->graphs.end()) {
+    hipGraph_t graph = nullptr;
+    FS3D_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_ticks(w, nticks, hash_every, w->d_small);
+    hipError_t e = hipStreamEndCapture(w->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    it = w->graphs.emplace(key, exec).first;
+  }
+  FS3D_HIP(launch_set_u64(w->d_small, w->tick, w->stream));
+  FS3D_HIP(hipEventRecord(w->ev0, w->stream));
+  FS3D_HIP(hipGraphLaunch(it->second, w->stream));
+  FS3D_HIP(hipEventRecord(w->ev1, w->stream));
+  if (nh > 0)
+    FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, (size_t)nh * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  float ms = 0.f;
+  FS3D_HIP(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  w->last_run_ms = (double)ms / nticks;
+  for (int i = 0; i < nh; i++) hashes[i] = w->h_pinned[i];
+  w->cur = (w->cur + nticks) & 1;
+  w->tick += (u64)nticks;
+  return clear_error();
+}
+
+int fs3d_world_last_run_ms(const fs3d_world *w, double *ms) {
+  if (int rc = check_world(w)) return rc;
+  if (!ms) return set_error(FS3D_ERR_INVALID, "ms is NULL");
+  *ms = w->last_run_ms;
+  return clear_error();
+}
+
+int fs3d_world_counts(fs3d_world *w, uint64_t *counts) {
+  if (int rc = check_world(w)) return rc;
+  if (!counts) return set_error(FS3D_ERR_INVALID, "counts is NULL");
+  FS3D_HIP(hipSetDevice(w->dev));
+  u64 *d = nullptr;
+  FS3D_HIP(hipMallocAsync((void **)&d, 256 * sizeof(u64), w->stream));
+  FS3D_HIP(hipMemsetAsync(d, 0, 256 * sizeof(u64), w->stream));
+  FS3D_HIP(launch_counts(w->buf[w->cur], w->n, d, w->stream));
+  FS3D_HIP(hipMemcpyAsync(counts, d, 256 * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipFreeAsync(d, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_sync(fs3d_world *w) {
+  if (int rc = check_world(w)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_device_ptr(fs3d_world *w, const void **dptr) {
+  if (int rc = check_world(w)) return rc;
+  if (!dptr) return set_error(FS3D_ERR_INVALID, "dptr is NULL");
+  *dptr = w->buf[w->cur];
+  return clear_error();
+}
+
+int fs3d_world_fill_rain(fs3d_world *w, uint64_t seed, uint64_t thr) {
+  if (int rc = check_world(w)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(launch_fill_rain(w->buf[w->cur], w->nx, w->ny, w->nz, seed, thr, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+int fs3d_world_fill_boxes(fs3d_world *w, uint64_t seed, const fs3d_box_params *p, int64_t *nboxes) {
+  if (int rc = check_world(w)) return rc;
+  if (!p) return set_error(FS3D_ERR_INVALID, "params is NULL");
+  FS3D_HIP(hipSetDevice(w->dev));
+  u8 *host = nullptr;
+  FS3D_HIP(hipHostMalloc(&host, w->n, hipHostMallocDefault));
+  const long long nb = paint_boxes(host, w->nx, w->ny, w->nz, seed, *p);
+  if (nb < 0) {
+    (void)hipHostFree(host);
+    return set_error(FS3D_ERR_INVALID, "box parameters do not fit the grid");
+  }
+  hipError_t e = hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  (void)hipHostFree(host);
+  if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "box upload failed: %s", hipGetErrorString(e));
+  if (nboxes) *nboxes = nb;
+  return clear_error();
+}
+
+int fs3d_world_fill_pillars(fs3d_world *w, uint64_t seed, const fs3d_pillar_params *p) {
+  if (int rc = check_world(w)) return rc;
+  if (!p) return set_error(FS3D_ERR_INVALID, "params is NULL");
+  int pool[5];
+  std::vector<Pillar> pillars;
+  if (pillar_list(w->nx, w->ny, w->nz, seed, *p, pool, &pillars) != 0)
+    return set_error(FS3D_ERR_INVALID, "pillar parameters do not fit the grid");
+  FS3D_HIP(hipSetDevice(w->dev));
+  u8 *g = w->buf[w->cur];
+  const size_t layer = (size_t)w->nx * w->ny;
+  FS3D_HIP(hipMemsetAsync(g, FS3D_EMPTY, w->n, w->stream));
+  FS3D_HIP(hipMemsetAsync(g, FS3D_STATIC, layer, w->stream));
+  if (pool[4] >= 1)
+    FS3D_HIP(launch_fill_box(g, w->nx, w->ny, pool[0], pool[1], 1, pool[2], pool[3], pool[4], FS3D_FLUID, w->stream));
+  if (!pillars.empty()) {
+    Pillar *d = nullptr;
+    int max_r = 0, max_zt = 0;
+    for (const Pillar &q : pillars) max_r = std::max(max_r, q.r), max_zt = std::max(max_zt, q.zt);
+    FS3D_HIP(hipMalloc(&d, pillars.size() * sizeof(Pillar)));
+    hipError_t e = hipMemcpyAsync(d, pillars.data(), pillars.size() * sizeof(Pillar), hipMemcpyHostToDevice, w->stream);
+    if (e == hipSuccess) e = launch_fill_pillars(g, w->nx, w->ny, d, (int)pillars.size(), max_r, max_zt, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "pillar fill failed: %s", hipGetErrorString(e));
+  }
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return clear_error();
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Internal declarations shared by the fs3d library sources (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "../../include/fs3d.h"
+
+namespace fs3d {
+
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr u64 kGamma = 0x9E3779B97F4A7C15ull;
+// Hash partial sums: NPART slots, one 64-byte line apart, summed by a finaliser.
+constexpr int kHashParts = 1024;
+constexpr int kHashStride = 8;  // u64 per slot
+// Bytes allocated past the end of every grid buffer: tick loads may read up to
+// one lane segment (<= 64 bytes) beyond the last row.
+constexpr size_t kGuardBytes = 256;
+constexpr int kMaxNx = 4096;  // 64 lanes x 64-byte segments per row
+
+int set_error(int code, const char *fmt, ...);
+int clear_error();
+
+#define FS3D_HIP(expr)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::fs3d::set_error(FS3D_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- host math shared by the runtime ------------------------------------------
+u64 mix64(u64 z);
+u64 sm64_next(u64 *state);
+u64 mulhi64(u64 a, u64 b);
+u64 uniform_incl(u64 r, u64 lo, u64 hi);
+u64 prob_threshold(u64 num, u64 den);
+int loguniform_int(u64 r, int lo, int hi);
+
+// ---- rule table ------------------------------------------------------------------
+// entries[v*65536 + s]: byte b of the u64 is the output category of cell b.
+void build_rule_entries(u64 *entries);
+u16 rule_state_packed(u16 s, int v);
+
+// ---- device context -------------------------------------------------------------
+int ensure_device(int dev);            // sets the device current, uploads the rule table
+const u64 *device_rule_table(int dev);  // valid after ensure_device
+
+// ---- tick kernels -----------------------------------------------------------------
+struct TickShape {
+  int seg;            // bytes per lane along x
+  int lanes_per_row;  // ceil(nx / seg) (<= 64)
+  int rq_per_wave;    // row quads (2 rows x 2 layers) per 64-lane wave
+  long long waves;    // waves covering the larger parity's row-quad count
+  unsigned blocks;    // workgroups of 4 waves
+};
+int plan_tick(int nx, int ny, int nz, int seg_req, TickShape *out);
+hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz,
+                       u32 t, const u64 *tick_dev, u32 k_off, const u64 *rule,
+                       u64 *hash_parts, hipStream_t s);
+hipError_t launch_hash(const u8 *g, size_t n, u64 *hash_parts, hipStream_t s);
+hipError_t launch_hash_finalize(u64 *hash_parts, u64 *out, hipStream_t s);
+hipError_t launch_set_u64(u64 *p, u64 v, hipStream_t s);
+hipError_t launch_counts(const u8 *g, size_t n, u64 *counts, hipStream_t s);
+
+// ---- scene kernels ----------------------------------------------------------------
+hipError_t launch_fill_rain(u8 *g, int nx, int ny, int nz, u64 seed, u64 thr, hipStream_t s);
+struct Pillar {
+  int cx, cy, r, zt;
+};
+hipError_t launch_fill_box(u8 *g, int nx, int ny, int x0, int y0, int z0, int sx, int sy, int sz,
+                           u8 v, hipStream_t s);
+hipError_t launch_fill_pillars(u8 *g, int nx, int ny, const Pillar *dev_pillars, int n,
+                               int max_r, int max_zt, hipStream_t s);
+// Host box painter; returns the number of boxes or -1 on bad parameters.
+long long paint_boxes(u8 *g, int nx, int ny, int nz, u64 seed, const fs3d_box_params &p);
+int pillar_list(int nx, int ny, int nz, u64 seed, const fs3d_pillar_params &p,
+                int pool[5] /* x0,y0,W,D,ztop */, std::vector<Pillar> *out);
+
+}  // namespace fs3d
+
+// ---- world object ----------------------------------------------------------------
+struct fs3d_world {
+  int dev = 0;
+  int nx = 0, ny = 0, nz = 0;
+  size_t n = 0;      // voxels
+  size_t alloc = 0;  // bytes per buffer (voxels + guard, rounded)
+  fs3d::u8 *buf[2] = {nullptr, nullptr};
+  int cur = 0;  // index of the buffer holding the current state
+  fs3d::u64 tick = 0;
+  hipStream_t stream = nullptr;
+  fs3d::u64 *d_parts = nullptr;  // hash partials
+  fs3d::u64 *d_small = nullptr;  // [0] device tick, [1..] hash outputs
+  int small_cap = 0;             // number of hash output slots
+  fs3d::u64 *h_pinned = nullptr; // pinned staging for small read-backs
+  int h_pinned_cap = 0;
+  fs3d::TickShape shape{};
+  int seg_req = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_run_ms = 0.0;
+  // cached graphs keyed by (nticks, hash_every, cur)
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+};

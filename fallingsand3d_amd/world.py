@@ -1,0 +1,204 @@
+"""Host-side mirror of the fs3d world interface (Python, over the C ABI).
+
+The reference engine exposes init/run/cleanup (src/engine/engine.h:16-35) and a
+Renderer with init/draw/cleanup (src/engine/rendering/renderer.h:81-98); it has
+no voxel world.  `World` is the object its run() loop would step between event
+handling and drawing (src/engine/engine.cpp:59-69): it owns a uint8 grid
+v[z][y][x] (z up) resident on the GPU, double-buffered, and every method maps
+one-to-one onto an `fs3d_*` entry point of include/fs3d.h.  Errors raise
+`Fs3dError` (the reference logs and throws std::runtime_error,
+src/util/debug.cpp:23-27).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import (BoxParams, Fs3dError, PillarParams, c_double, c_i32, c_i64, c_int, c_u8, c_u64,
+                   check, lib, ptr, vp)
+
+EMPTY, STATIC, SAND, FLUID = 0, 1, 2, 3
+
+# Scene parameters of the measured workloads.
+BOXES_512 = dict(side_min=8, side_max=64, pct=(20, 10, 5), max_boxes=1 << 20)
+PILLARS_2048 = (256, 4, 32, 256, 1800, 256, 682, 256, 682, 32, 128)
+
+
+def device_count():
+    n = c_int(0)
+    check(lib().fs3d_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def init(device=0):
+    check(lib().fs3d_init(device))
+
+
+def rule_lut():
+    """The packed block rule lut[v*65536 + s] (host-side table, no GPU needed)."""
+    lut = np.zeros(4 * 65536, np.uint16)
+    check(lib().fs3d_rule_lut(ptr(lut, ctypes.c_uint16)))
+    return lut
+
+
+def threshold(num, den):
+    t = c_u64(0)
+    check(lib().fs3d_threshold(num, den, ctypes.byref(t)))
+    return t.value
+
+
+def batch_specs(n, seed, lo=16, hi=128):
+    dims = np.zeros(3 * n, np.int32)
+    thr = np.zeros(n, np.uint64)
+    seeds = np.zeros(n, np.uint64)
+    check(lib().fs3d_batch_specs(n, seed, lo, hi, ptr(dims, c_i32), ptr(thr, c_u64), ptr(seeds, c_u64)))
+    return dims.reshape(n, 3), thr, seeds
+
+
+def serve_specs(n, seed, lo=32, hi=256, tick_lo=10, tick_hi=500, lam=2000.0):
+    dims = np.zeros(3 * n, np.int32)
+    thr = np.zeros(n, np.uint64)
+    seeds = np.zeros(n, np.uint64)
+    ticks = np.zeros(n, np.int32)
+    arr = np.zeros(n, np.float64)
+    check(lib().fs3d_serve_specs(n, seed, lo, hi, tick_lo, tick_hi, lam, ptr(dims, c_i32), ptr(thr, c_u64),
+                                 ptr(seeds, c_u64), ptr(ticks, c_i32), ptr(arr, c_double)))
+    return dims.reshape(n, 3), thr, seeds, ticks, arr
+
+
+class World:
+    """A uint8 voxel grid of dims (nx, ny, nz) on the current GPU."""
+
+    def __init__(self, dims, flags=0):
+        self.dims = tuple(int(d) for d in dims)
+        h = vp()
+        d = (c_i32 * 3)(*self.dims)
+        check(lib().fs3d_world_create(d, flags, ctypes.byref(h)))
+        self._h = h
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fs3d_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise Fs3dError(6, "world is closed")
+        return self._h
+
+    @property
+    def shape(self):
+        nx, ny, nz = self.dims
+        return (nz, ny, nx)
+
+    @property
+    def size(self):
+        nx, ny, nz = self.dims
+        return nx * ny * nz
+
+    # -- grid access ---------------------------------------------------------
+    def set_voxel(self, x, y, z, v):
+        check(lib().fs3d_world_set_voxel(self.handle, x, y, z, v))
+
+    def get_voxel(self, x, y, z):
+        v = c_u8(0)
+        check(lib().fs3d_world_get_voxel(self.handle, x, y, z, ctypes.byref(v)))
+        return v.value
+
+    def upload(self, grid):
+        g = np.ascontiguousarray(grid, dtype=np.uint8)
+        if g.shape != self.shape:
+            raise Fs3dError(1, f"grid shape {g.shape} != {self.shape} (z, y, x)")
+        check(lib().fs3d_world_upload(self.handle, g.ctypes.data, g.size))
+
+    def download(self):
+        g = np.empty(self.shape, np.uint8)
+        check(lib().fs3d_world_download(self.handle, g.ctypes.data, g.size))
+        return g
+
+    @property
+    def tick_count(self):
+        t = c_u64(0)
+        check(lib().fs3d_world_get_tick(self.handle, ctypes.byref(t)))
+        return t.value
+
+    @tick_count.setter
+    def tick_count(self, t):
+        check(lib().fs3d_world_set_tick(self.handle, t))
+
+    # -- tick ----------------------------------------------------------------
+    def step(self, nticks=1):
+        check(lib().fs3d_world_step(self.handle, nticks))
+
+    def tick(self):
+        """One tick; returns the hash of the new state (fused into the kernel)."""
+        h = c_u64(0)
+        check(lib().fs3d_world_tick(self.handle, ctypes.byref(h)))
+        return h.value
+
+    def hash(self):
+        h = c_u64(0)
+        check(lib().fs3d_world_hash(self.handle, ctypes.byref(h)))
+        return h.value
+
+    def run(self, nticks, hash_every=0):
+        """nticks ticks as one captured HIP graph; returns the periodic hashes."""
+        nh = nticks // hash_every if hash_every > 0 else 0
+        out = np.zeros(max(nh, 1), np.uint64)
+        check(lib().fs3d_world_run(self.handle, nticks, hash_every, ptr(out, c_u64)))
+        return out[:nh]
+
+    def last_run_ms_per_tick(self):
+        ms = c_double(0)
+        check(lib().fs3d_world_last_run_ms(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def counts(self):
+        c = np.zeros(256, np.uint64)
+        check(lib().fs3d_world_counts(self.handle, ptr(c, c_u64)))
+        return c
+
+    def sync(self):
+        check(lib().fs3d_world_sync(self.handle))
+
+    def device_ptr(self):
+        p = vp()
+        check(lib().fs3d_world_device_ptr(self.handle, ctypes.byref(p)))
+        return p.value
+
+    @property
+    def segment(self):
+        s = c_i32(0)
+        check(lib().fs3d_world_get_segment(self.handle, ctypes.byref(s)))
+        return s.value
+
+    @segment.setter
+    def segment(self, s):
+        check(lib().fs3d_world_set_segment(self.handle, s))
+
+    # -- scenes ----------------------------------------------------------------
+    def fill_rain(self, seed, thr):
+        check(lib().fs3d_world_fill_rain(self.handle, seed, thr))
+
+    def fill_boxes(self, seed, side_min=8, side_max=64, pct=(20, 10, 5), max_boxes=1 << 20):
+        p = BoxParams(side_min, side_max, pct[0], pct[1], pct[2], max_boxes)
+        n = c_i64(0)
+        check(lib().fs3d_world_fill_boxes(self.handle, seed, ctypes.byref(p), ctypes.byref(n)))
+        return n.value
+
+    def fill_pillars(self, seed, params=PILLARS_2048):
+        p = PillarParams(*params)
+        check(lib().fs3d_world_fill_pillars(self.handle, seed, ctypes.byref(p)))

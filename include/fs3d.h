@@ -1,0 +1,141 @@
+/*
+ * fs3d.h -- C ABI of the MI355X-native (gfx950) falling-sand voxel library.
+ *
+ * Plain C types only (pointers, sizes, fixed-width integers).  Every call returns
+ * an fs3d_status; on failure fs3d_last_error() returns a thread-local message.
+ *
+ * The reference (amarde05/FallingSand3D) is one monolithic Vulkan executable with
+ * no plugin, operator or FFI API (src/CMakeLists.txt:1-8) and no voxel grid.  The
+ * seams this ABI replaces are named per entry point:
+ *   - the engine loop call site, where a world step belongs between event
+ *     handling and drawing (src/engine/engine.cpp:59-69);
+ *   - the renderer's hand-off of data to shaders through per-material set-2
+ *     bindings (MaterialLayout::addDataBinding / Material::writeBuffer,
+ *     src/engine/rendering/materials.cpp:26-59, :388-412) and the fs_raymarch
+ *     full-screen draw (shaders/fs_raymarch.frag:67-81);
+ *   - the GlobalData camera block written each frame
+ *     (src/engine/rendering/materials.cpp:531-575, materials.h:19-28);
+ *   - error reporting, which in the reference logs and throws
+ *     (util::displayError, src/util/debug.cpp:23-27); here codes + message, and
+ *     include/fs3d.hpp rethrows as std::runtime_error.
+ *
+ * Threading: calls on different handles may run concurrently; calls on one
+ * handle must be serialised by the caller.  fs3d_world_step is asynchronous on
+ * the world's HIP stream; hash, download, get_voxel, run and render synchronise.
+ */
+#ifndef FS3D_H
+#define FS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS3D_ABI_VERSION 1
+
+typedef enum fs3d_status {
+  FS3D_OK = 0,
+  FS3D_ERR_INVALID = 1,   /* bad argument (dims, coordinates, sizes, NULL) */
+  FS3D_ERR_NOMEM = 2,     /* device or host allocation failed */
+  FS3D_ERR_BUDGET = 3,    /* request exceeds the configured HBM budget */
+  FS3D_ERR_HIP = 4,       /* a HIP runtime call failed */
+  FS3D_ERR_NODEVICE = 5,  /* no usable gfx950 device */
+  FS3D_ERR_STATE = 6      /* call not valid in the object's current state */
+} fs3d_status;
+
+/* Material bytes. 4..255 are reserved: they behave as STATIC and keep their value. */
+enum { FS3D_EMPTY = 0, FS3D_STATIC = 1, FS3D_SAND = 2, FS3D_FLUID = 3 };
+
+typedef struct fs3d_world fs3d_world; /* opaque */
+
+/* ---- library ------------------------------------------------------------- */
+/* Replaces: util::displayError's message (src/util/debug.cpp:23-27). */
+const char *fs3d_last_error(void);
+int fs3d_abi_version(void);
+/* Number of visible HIP devices (0 without a GPU; never an error for that). */
+int fs3d_device_count(int *count);
+/* Selects the device for this thread and uploads the rule table to it.  Called
+ * implicitly with device 0 by the first world creation.  Replaces the device
+ * pick in VulkanDevice (src/engine/rendering/device.cpp:172-201). */
+int fs3d_init(int device);
+/* The packed block rule, lut[v*65536 + s] = R(s, v) (2 bits per cell); host
+ * only, no device needed.  Exposes the table the tick kernels read. */
+int fs3d_rule_lut(uint16_t *lut /* 4*65536 */);
+
+/* ---- world grid ------------------------------------------------------------ */
+/* uint8 v[z][y][x], x fastest, z up.  No reference counterpart exists (the
+ * reference Renderer holds meshes, materials and a camera only,
+ * src/engine/rendering/renderer.h:100-154). */
+int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out);
+int fs3d_world_destroy(fs3d_world *w); /* NULL is a no-op */
+int fs3d_world_dims(const fs3d_world *w, int32_t dims[3]);
+int fs3d_world_set_voxel(fs3d_world *w, int32_t x, int32_t y, int32_t z, uint8_t v);
+int fs3d_world_get_voxel(fs3d_world *w, int32_t x, int32_t y, int32_t z, uint8_t *v);
+int fs3d_world_upload(fs3d_world *w, const uint8_t *host, size_t nbytes);
+int fs3d_world_download(fs3d_world *w, uint8_t *host, size_t nbytes);
+int fs3d_world_get_tick(const fs3d_world *w, uint64_t *tick);
+int fs3d_world_set_tick(fs3d_world *w, uint64_t tick);
+
+/* ---- tick ------------------------------------------------------------------ */
+/* Attaches at the engine loop seam (src/engine/engine.cpp:59-69): advances the
+ * world by nticks ticks; asynchronous. */
+int fs3d_world_step(fs3d_world *w, int32_t nticks);
+/* One tick with the state hash of the new state fused into the same kernel. */
+int fs3d_world_tick(fs3d_world *w, uint64_t *hash);
+/* State hash of the current grid (synchronises). */
+int fs3d_world_hash(fs3d_world *w, uint64_t *hash);
+/* nticks ticks as one captured HIP graph (cached per shape), writing the hash
+ * after every hash_every-th tick to hashes[nticks / hash_every] (hash_every = 0:
+ * none).  Synchronises. */
+int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *hashes);
+/* Per-material cell counts of the current grid (counts[256]); synchronises. */
+int fs3d_world_counts(fs3d_world *w, uint64_t *counts);
+/* Blocks until all work queued on the world's stream has finished. */
+int fs3d_world_sync(fs3d_world *w);
+/* Device pointer of the current grid, for a renderer on the same device (the
+ * set-2 data binding hand-off, src/engine/rendering/materials.cpp:388-412).
+ * Valid until the next step/run/tick on this world. */
+int fs3d_world_device_ptr(fs3d_world *w, const void **dptr);
+
+/* Tick kernel launch shape: bytes per lane along x (8, 16, 32 or 64; 0 = auto). */
+int fs3d_world_set_segment(fs3d_world *w, int32_t seg_bytes);
+int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg_bytes);
+
+/* ---- scenes (seeded, bit-exact, generated by this library) ------------------- */
+/* Rain fill: STATIC floor at z = 0; cells with z >= max(1, nz/2) are SAND when
+ * cell_rand(seed, i) < thr.  Generated on the device. */
+int fs3d_world_fill_rain(fs3d_world *w, uint64_t seed, uint64_t thr);
+typedef struct fs3d_box_params {
+  int32_t side_min, side_max;
+  int32_t pct_sand, pct_fluid, pct_static;
+  int32_t max_boxes;
+} fs3d_box_params;
+/* Seeded axis-aligned boxes over a floor, painted until target fractions are
+ * met.  Generated on the host and uploaded.  *nboxes receives the box count. */
+int fs3d_world_fill_boxes(fs3d_world *w, uint64_t seed, const fs3d_box_params *p, int64_t *nboxes);
+typedef struct fs3d_pillar_params {
+  int32_t n_pillars, r_min, r_max, h_min, h_max;
+  int32_t pool_w_min, pool_w_max, pool_d_min, pool_d_max, pool_h_min, pool_h_max;
+} fs3d_pillar_params;
+/* Seeded sand pillars standing in a fluid pool; generated on the device. */
+int fs3d_world_fill_pillars(fs3d_world *w, uint64_t seed, const fs3d_pillar_params *p);
+/* Host-side spec streams for batched worlds and serving requests. */
+int fs3d_batch_specs(int32_t n, uint64_t seed, int32_t lo, int32_t hi, int32_t *dims,
+                     uint64_t *thr, uint64_t *seeds);
+int fs3d_serve_specs(int32_t n, uint64_t seed, int32_t lo, int32_t hi, int32_t tick_lo,
+                     int32_t tick_hi, double lam, int32_t *dims, uint64_t *thr, uint64_t *seeds,
+                     int32_t *ticks, double *arrival);
+int fs3d_threshold(uint64_t num, uint64_t den, uint64_t *thr);
+
+/* ---- timing ---------------------------------------------------------------- */
+/* Average device time (ms) of the tick kernels launched by the last
+ * fs3d_world_run on this world, measured with HIP events on the world's stream
+ * (first launch to last launch, divided by the tick count). */
+int fs3d_world_last_run_ms(const fs3d_world *w, double *ms_per_tick);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
