@@ -223,11 +223,15 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   }
 
   // ---- variants: one 32-bit word per 16 consecutive blocks ----
+  // A lane evaluates blocks bx0 .. bx0 + S/2 (the last one on odd ticks only);
+  // with bx0 & 15 = jb they span groups g0 .. g0 + (jb + S/2) / 16.
+  constexpr int NG = S >= 64 ? 3 : 2;
   const u32 bx0 = (u32)(seg * (S / 2));
   const u32 g0 = bx0 >> 4;
   const u32 yzt = (u32)yq * 0x85EBCA77u + (u32)zq * 0xC2B2AE3Du + t * 0x27D4EB2Fu;
-  const u32 w0 = fmix32(g0 * 0x9E3779B1u + yzt);
-  const u32 w1 = fmix32((g0 + 1u) * 0x9E3779B1u + yzt);
+  u32 wg[NG];
+#pragma unroll
+  for (int i = 0; i < NG; i++) wg[i] = fmix32((g0 + (u32)i) * 0x9E3779B1u + yzt);
   const u32 jb = (S >= 32) ? 0u : (bx0 & 15u);
 
   // ---- evaluate blocks (two per frame dword) and scatter outputs ----
@@ -251,7 +255,9 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
         continue;
       }
       const u32 kb = jb + (u32)(2 * j + h);
-      const u32 w = (kb >> 4) ? w1 : w0;
+      u32 w = wg[0];
+#pragma unroll
+      for (int i = 1; i < NG; i++) w = (kb >> 4) == (u32)i ? wg[i] : w;
       const u32 v = (w >> (2u * (kb & 15u))) & 3u;
       const u32 s = h ? (states >> 16) : (states & 0xFFFFu);
       e[h] = a.rule[(v << 16) | s];
