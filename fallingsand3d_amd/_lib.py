@@ -78,6 +78,7 @@ SIGNATURES = {
                                  P(c_u64), P(c_u64), P(c_i32), P(c_double)]),
     "fs3d_threshold": (c_int, [c_u64, c_u64, P(c_u64)]),
     "fs3d_world_last_run_ms": (c_int, [vp, P(c_double)]),
+    "fs3d_copy_probe": (c_int, [c_size, c_i32, P(c_double)]),
 }
 
 _LIB = None

@@ -40,6 +40,13 @@ def rule_lut():
     return lut
 
 
+def copy_probe(nbytes, iters=20):
+    """Mean ms of a plain device copy of nbytes (bandwidth calibration)."""
+    ms = c_double(0)
+    check(lib().fs3d_copy_probe(nbytes, iters, ctypes.byref(ms)))
+    return ms.value
+
+
 def threshold(num, den):
     t = c_u64(0)
     check(lib().fs3d_threshold(num, den, ctypes.byref(t)))

@@ -134,6 +134,11 @@ int fs3d_threshold(uint64_t num, uint64_t den, uint64_t *thr);
  * fs3d_world_run on this world, measured with HIP events on the world's stream
  * (first launch to last launch, divided by the tick count). */
 int fs3d_world_last_run_ms(const fs3d_world *w, double *ms_per_tick);
+/* Mean device time (ms) of a plain streaming copy of nbytes (16-byte vector
+ * loads and stores per lane): the practical bandwidth ceiling the tick is
+ * compared with, and the known-size kernel that scales rocprof byte counters.
+ * No reference counterpart (the reference has no timing at all). */
+int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
