@@ -77,6 +77,48 @@ def bench_copy(F, nbytes, iters=20):
     return dict(nbytes=nbytes, ms=ms, GBps=2 * nbytes / (ms * 1e-3) / 1e9)
 
 
+def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
+    """4096 worlds, edges log-uniform in [16, 128], rain fill with per-world
+    density in [0.05, 0.5): all worlds step `ticks` ticks in one persistent
+    launch (shape chosen by the autotuner) against the plain loop of one launch
+    per world per tick.  The plain loop is timed on its first `plain_ticks`
+    ticks (2 x 4096 x 20 launches would not change its per-tick cost) and its
+    hashes are compared with the persistent kernel's over the same ticks."""
+    dims, thr, seeds = F.batch_specs(nworlds, seed, 16, 128)
+    vox = int(dims.astype("int64").prod(1).sum())
+    with F.Batch(dims) as b:
+        b.fill_rain(seeds, thr)
+        tune = b.autotune(trial_ticks=10)
+        log(f"[bench] batched: autotune -> seg {tune['seg']} threads {tune['threads']}")
+        h_sample = b.run(plain_ticks)
+        with F.Batch(dims) as p:
+            p.fill_rain(seeds, thr)
+            t0 = time.perf_counter()
+            h_plain = p.run(plain_ticks, plain=True)
+            plain_wall = time.perf_counter() - t0
+        exact = bool((h_sample == h_plain).all())
+        log(f"[bench] batched: plain loop {plain_ticks} ticks in {plain_wall:.3f}s, bit-exact={exact}")
+        b.tick_count = 0
+        b.fill_rain(seeds, thr)
+        b.run(ticks)  # graph capture + warm caches
+        b.tick_count = 0
+        b.fill_rain(seeds, thr)
+        t0 = time.perf_counter()
+        h = b.run(ticks)
+        wall = time.perf_counter() - t0
+        dev_ms = b.last_run_ms()
+        log(f"[bench] batched: persistent {ticks} ticks in {wall * 1e3:.1f} ms (device {dev_ms:.1f} ms)")
+        digest = 0
+        for v in h:
+            digest = (digest + int(v)) % (1 << 64)
+        plain_per_tick = plain_wall / plain_ticks
+        return dict(nworlds=nworlds, vox=vox, ticks=ticks, wall=wall, dev_ms=dev_ms,
+                    gvox_per_s=vox * ticks / wall / 1e9, plain_ms_per_tick=plain_per_tick * 1e3,
+                    plain_gvox_per_s=vox / plain_per_tick / 1e9,
+                    speedup=(plain_per_tick * ticks) / wall, bit_exact_sample=exact,
+                    tuned={"seg": tune["seg"], "threads": tune["threads"]}, hash_digest=digest)
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's tick (table-driven, one thread) on the 512^3 box scene, timed
     for about `seconds` of host work."""
@@ -99,7 +141,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy"], default=None)
+    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched"], default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,6 +161,10 @@ def main():
     if args.only == "copy":
         c = bench_copy(F, 8 << 30)
         print(json.dumps({"workload": "copy", **c}), flush=True)
+        return
+    if args.only == "batched":
+        bb = bench_batched(F, ticks=args.steps)
+        print(json.dumps({"workload": "batched", **bb}), flush=True)
         return
 
     r = bench_boxes512(F, args.steps, args.warmup)
@@ -159,6 +205,14 @@ def main():
             "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4)}
+        bb = bench_batched(F)
+        extra["batched"] = {
+            "workload": "4096 worlds, edges log-uniform 16-128, rain fill, density U(0.05,0.5), 500 ticks",
+            "voxels": bb["vox"], "gvox_per_s": round(bb["gvox_per_s"], 2), "wall_ms": round(bb["wall"] * 1e3, 2),
+            "device_ms": round(bb["dev_ms"], 2), "plain_gvox_per_s": round(bb["plain_gvox_per_s"], 3),
+            "plain_ms_per_tick_sampled": round(bb["plain_ms_per_tick"], 3),
+            "speedup_vs_plain": round(bb["speedup"], 2), "bit_exact_sample": bb["bit_exact_sample"],
+            "autotuned": bb["tuned"], "final_hash_digest": bb["hash_digest"]}
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -79,6 +79,19 @@ SIGNATURES = {
     "fs3d_threshold": (c_int, [c_u64, c_u64, P(c_u64)]),
     "fs3d_world_last_run_ms": (c_int, [vp, P(c_double)]),
     "fs3d_copy_probe": (c_int, [c_size, c_i32, P(c_double)]),
+    "fs3d_batch_create": (c_int, [c_i32, P(c_i32), P(vp)]),
+    "fs3d_batch_destroy": (c_int, [vp]),
+    "fs3d_batch_size": (c_int, [vp, P(c_i32)]),
+    "fs3d_batch_upload": (c_int, [vp, c_i32, vp, c_size]),
+    "fs3d_batch_download": (c_int, [vp, c_i32, vp, c_size]),
+    "fs3d_batch_fill_rain": (c_int, [vp, P(c_u64), P(c_u64)]),
+    "fs3d_batch_get_tick": (c_int, [vp, P(c_u64)]),
+    "fs3d_batch_set_tick": (c_int, [vp, c_u64]),
+    "fs3d_batch_set_config": (c_int, [vp, c_i32, c_i32]),
+    "fs3d_batch_get_config": (c_int, [vp, P(c_i32), P(c_i32), P(c_i32)]),
+    "fs3d_batch_run": (c_int, [vp, c_i32, c_i32, P(c_u64)]),
+    "fs3d_batch_last_run_ms": (c_int, [vp, P(c_double)]),
+    "fs3d_batch_autotune": (c_int, [vp, c_i32, P(c_i32), P(c_i32), P(c_double)]),
 }
 
 _LIB = None

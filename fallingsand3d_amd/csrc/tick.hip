@@ -5,139 +5,16 @@
 // reference has no tick; its engine loop (src/engine/engine.cpp:59-69) is the
 // call site this replaces.
 //
-// Work decomposition.  A "row quad" is the 4 rows (y, y+1) x (z, z+1) of one block
-// row: every block of a row quad reads and writes only those 4 rows, so each row
-// quad is independent.  Each lane owns a segment of S bytes of the 4 rows
-// (lanes_per_row = ceil(nx / S) <= 64 lanes per row quad); a wave packs
-// floor(64 / lanes_per_row) row quads.  All loads and stores are S-byte vector
-// accesses, coalesced across the lanes of a row, so the tick streams 1 byte in
-// and 1 byte out per voxel.
-//
-// Odd parity shifts the blocks by one byte against the lane segments: every lane
-// takes the last byte of its left neighbour and the first byte of its right
-// neighbour through full-wave DPP shifts (wave_shr:1 / wave_shl:1), evaluates
-// S/2 + 1 blocks on the shifted frame, and keeps the outputs of its own bytes.
-// The straddling block is evaluated by both lanes from identical inputs.
-//
-// Block evaluation is a table lookup.  The 4 rows' 2-bit categories of a block are
-// packed into a 16-bit state with SWAR byte operations (two blocks per dword);
-// the table (4 variants x 65536 states x 8 bytes, 2 MiB, L2-resident) returns
-// the 8 output categories as bytes, which v_perm scatters back into row dwords.
-// Reserved bytes (>= 4) take a slower wave-uniform path that maps them to STATIC
-// and restores their value afterwards.
-//
-// The optional fused hash adds mix64((i << 8) | v) for every non-zero output
-// cell into one of kHashParts partial sums (64-byte apart), summed later by
-// hash_finalize.
-#include "fs3d_internal.h"
+// The per-wave work (row quads, DPP exchange, table lookup) lives in
+// tick_core.h; this file holds the single-world launches, the hash passes and
+// small helper kernels.
+#include "tick_core.h"
 
 namespace fs3d {
 
 namespace {
 
-typedef u32 u32x2 __attribute__((ext_vector_type(2)));
-typedef u32 u32x4 __attribute__((ext_vector_type(4)));
-// Row starts are arbitrary byte offsets; gfx9 global accesses do not require
-// natural alignment, so segments are moved with unaligned vector accesses.
-typedef u32x2 __attribute__((aligned(1))) u32x2_u;
-typedef u32x4 __attribute__((aligned(1))) u32x4_u;
-
-constexpr u32 kStatic4 = 0x01010101u;
-
-__device__ __forceinline__ u32 lane_from_left(u32 v) {  // lane L receives lane L-1
-  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
-}
-__device__ __forceinline__ u32 lane_from_right(u32 v) {  // lane L receives lane L+1
-  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
-}
-
-// 0xFF in every byte whose value is >= 4 (reserved materials), else 0x00.
-__device__ __forceinline__ u32 reserved_mask(u32 x) {
-  const u32 nz = (((x & 0x7C7C7C7Cu) + 0x7C7C7C7Cu) | x) & 0x80808080u;
-  return (nz - (nz >> 7)) | nz;
-}
-
-// Categories: byte values 0..3 unchanged, reserved bytes -> STATIC (1).
-__device__ __forceinline__ u32 to_codes(u32 x, u32 rmask) {
-  return (x & 0x03030303u & ~rmask) | (rmask & kStatic4);
-}
-
-__device__ __forceinline__ u32 fmix32(u32 h) {
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
-  h ^= h >> 16;
-  return h;
-}
-
-__device__ __forceinline__ u64 dmix64(u64 z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Keeps the low `keep` bytes of x (0..4) and fills the others with STATIC.
-__device__ __forceinline__ u32 keep_bytes(u32 x, int keep) {
-  if (keep >= 4) return x;
-  if (keep <= 0) return kStatic4;
-  const u32 m = (1u << (8 * keep)) - 1u;
-  return (x & m) | (kStatic4 & ~m);
-}
-
-template <int S>
-__device__ __forceinline__ void load_seg(const u8 *p, u32 (&d)[S / 4]) {
-  if constexpr (S == 8) {
-    const u32x2 v = *reinterpret_cast<const u32x2_u *>(p);
-    d[0] = v.x;
-    d[1] = v.y;
-  } else {
-#pragma unroll
-    for (int c = 0; c < S / 16; c++) {
-      const u32x4 v = *reinterpret_cast<const u32x4_u *>(p + 16 * c);
-      d[4 * c + 0] = v.x;
-      d[4 * c + 1] = v.y;
-      d[4 * c + 2] = v.z;
-      d[4 * c + 3] = v.w;
-    }
-  }
-}
-
-template <int S>
-__device__ __forceinline__ void store_seg(u8 *p, const u32 (&d)[S / 4]) {
-  if constexpr (S == 8) {
-    u32x2 v;
-    v.x = d[0];
-    v.y = d[1];
-    *reinterpret_cast<u32x2_u *>(p) = v;
-  } else {
-#pragma unroll
-    for (int c = 0; c < S / 16; c++) {
-      u32x4 v;
-      v.x = d[4 * c + 0];
-      v.y = d[4 * c + 1];
-      v.z = d[4 * c + 2];
-      v.w = d[4 * c + 3];
-      *reinterpret_cast<u32x4_u *>(p + 16 * c) = v;
-    }
-  }
-}
-
-// Packs the 16-bit states of the two blocks held by one frame dword of the 4
-// rows: low half = block at bytes (0,1), high half = block at bytes (2,3).
-__device__ __forceinline__ u32 pack_states(u32 c0, u32 c1, u32 c2, u32 c3) {
-  const u32 r01 = c0 | (c1 << 4);
-  const u32 r23 = c2 | (c3 << 4);
-  const u32 a = r01 | (r01 >> 6);
-  const u32 b = r23 | (r23 >> 6);
-  return __builtin_amdgcn_perm(b, a, 0x06020400u);
-}
+using namespace tickcore;
 
 struct TickArgs {
   const u8 *src;
@@ -153,155 +30,16 @@ struct TickArgs {
 
 template <int S, bool HASH>
 __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
-  constexpr int ND = S / 4;  // dwords per lane per row
   const int lane = threadIdx.x & 63;
   const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const u32 t = a.tick_dev ? (u32)(*a.tick_dev + a.k_off) : a.t;
   const int p = (int)(t & 1u);
-  const int nyq = (a.ny + p + 1) >> 1, nzq = (a.nz + p + 1) >> 1;
-  const long long nrq = (long long)nyq * nzq;
+  const long long nrq = (long long)((a.ny + p + 1) >> 1) * ((a.nz + p + 1) >> 1);
   // Whole waves past the end leave early; the hashing variant keeps them until
   // its workgroup barrier (they load and store nothing).
   if (!HASH && wave * a.rq_per_wave >= nrq) return;
-
-  const int LR = a.lanes_per_row;
-  const int rql = lane / LR;
-  const int seg = lane - rql * LR;
-  const long long q = wave * a.rq_per_wave + rql;
-  const bool active = rql < a.rq_per_wave && q < nrq;
-  const int yq = active ? (int)(q % nyq) : 0;
-  const int zq = active ? (int)(q / nyq) : 0;
-  const int y0 = 2 * yq - p, z0 = 2 * zq - p;
-  const int xs = seg * S;
-  const int m = min(S, a.nx - xs);  // valid bytes of this segment
-
-  // ---- load the 4 rows (r = dy + 2*dz); out-of-grid rows read as STATIC ----
-  u32 own[4][ND];
-  size_t rowoff[4];
-  bool rvalid[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int y = y0 + (r & 1), z = z0 + (r >> 1);
-    rvalid[r] = active && y >= 0 && y < a.ny && z >= 0 && z < a.nz;
-    rowoff[r] = ((size_t)(rvalid[r] ? z : 0) * a.ny + (rvalid[r] ? y : 0)) * a.nx + xs;
-#pragma unroll
-    for (int j = 0; j < ND; j++) own[r][j] = kStatic4;
-    if (rvalid[r]) load_seg<S>(a.src + rowoff[r], own[r]);
-    if (m < S) {
-#pragma unroll
-      for (int j = 0; j < ND; j++) own[r][j] = keep_bytes(own[r][j], m - 4 * j);
-    }
-  }
-
-  // ---- reserved bytes: wave-uniform slow path ----
-  u32 anyres = 0;
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-#pragma unroll
-    for (int j = 0; j < ND; j++) anyres |= own[r][j] & 0xFCFCFCFCu;
-  const bool slow = __any(anyres != 0);
-
-  // ---- frame: even parity = own bytes; odd parity = [left byte, own, right byte] ----
-  constexpr int NF = ND + 1;  // frame dwords (the last is used by odd parity only)
-  u32 fr[4][NF];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const u32 from_l = lane_from_left(own[r][ND - 1]);
-    const u32 from_r = lane_from_right(own[r][0]);
-    if (p) {
-      const u32 lb = seg == 0 ? 0x01000000u : (from_l & 0xFF000000u);
-      const u32 rb = seg == LR - 1 ? 0x01u : (from_r & 0xFFu);
-      fr[r][0] = __builtin_amdgcn_alignbyte(own[r][0], lb, 3);
-#pragma unroll
-      for (int j = 1; j < ND; j++) fr[r][j] = __builtin_amdgcn_alignbyte(own[r][j], own[r][j - 1], 3);
-      fr[r][ND] = __builtin_amdgcn_alignbyte(rb, own[r][ND - 1], 3);
-    } else {
-#pragma unroll
-      for (int j = 0; j < ND; j++) fr[r][j] = own[r][j];
-      fr[r][ND] = kStatic4;
-    }
-  }
-
-  // ---- variants: one 32-bit word per 16 consecutive blocks ----
-  // A lane evaluates blocks bx0 .. bx0 + S/2 (the last one on odd ticks only);
-  // with bx0 & 15 = jb they span groups g0 .. g0 + (jb + S/2) / 16.
-  constexpr int NG = S >= 64 ? 3 : 2;
-  const u32 bx0 = (u32)(seg * (S / 2));
-  const u32 g0 = bx0 >> 4;
-  const u32 yzt = (u32)yq * 0x85EBCA77u + (u32)zq * 0xC2B2AE3Du + t * 0x27D4EB2Fu;
-  u32 wg[NG];
-#pragma unroll
-  for (int i = 0; i < NG; i++) wg[i] = fmix32((g0 + (u32)i) * 0x9E3779B1u + yzt);
-  const u32 jb = (S >= 32) ? 0u : (bx0 & 15u);
-
-  // ---- evaluate blocks (two per frame dword) and scatter outputs ----
-  u32 out[4][NF];
-#pragma unroll
-  for (int j = 0; j < NF; j++) {
-    if (j == ND && !p) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) out[r][j] = kStatic4;
-      continue;
-    }
-    u32 c[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) c[r] = slow ? to_codes(fr[r][j], reserved_mask(fr[r][j])) : fr[r][j];
-    const u32 states = pack_states(c[0], c[1], c[2], c[3]);
-    u64 e[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      if (j == ND && h == 1) {  // odd parity's last frame dword holds one block
-        e[1] = 0x0101010101010101ull;
-        continue;
-      }
-      const u32 kb = jb + (u32)(2 * j + h);
-      u32 w = wg[0];
-#pragma unroll
-      for (int i = 1; i < NG; i++) w = (kb >> 4) == (u32)i ? wg[i] : w;
-      const u32 v = (w >> (2u * (kb & 15u))) & 3u;
-      const u32 s = h ? (states >> 16) : (states & 0xFFFFu);
-      e[h] = a.rule[(v << 16) | s];
-    }
-    const u32 lo0 = (u32)e[0], hi0 = (u32)(e[0] >> 32), lo1 = (u32)e[1], hi1 = (u32)(e[1] >> 32);
-    out[0][j] = __builtin_amdgcn_perm(lo1, lo0, 0x05040100u);
-    out[1][j] = __builtin_amdgcn_perm(lo1, lo0, 0x07060302u);
-    out[2][j] = __builtin_amdgcn_perm(hi1, hi0, 0x05040100u);
-    out[3][j] = __builtin_amdgcn_perm(hi1, hi0, 0x07060302u);
-  }
-
-  // ---- back to own bytes, restore reserved values, store, hash ----
-  u64 acc = 0;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    u32 res[ND];
-#pragma unroll
-    for (int j = 0; j < ND; j++) {
-      u32 o = p ? __builtin_amdgcn_alignbyte(out[r][j + 1], out[r][j], 1) : out[r][j];
-      if (slow) {
-        const u32 rm = reserved_mask(own[r][j]);
-        o = (own[r][j] & rm) | (o & ~rm);
-      }
-      res[j] = o;
-    }
-    if (rvalid[r]) {
-      u8 *dp = a.dst + rowoff[r];
-      if (m == S) {
-        store_seg<S>(dp, res);
-      } else {
-#pragma unroll
-        for (int k = 0; k < S; k++)
-          if (k < m) dp[k] = (u8)(res[k >> 2] >> (8 * (k & 3)));
-      }
-      if constexpr (HASH) {
-#pragma unroll
-        for (int k = 0; k < S; k++) {
-          const u32 v = (res[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-          if (k < m && v) acc += dmix64(((u64)(rowoff[r] + k) << 8) | v);
-        }
-      }
-    }
-  }
-
+  u64 acc = tick_wave<S, HASH>(a.src, a.dst, a.nx, a.ny, a.nz, a.lanes_per_row, a.rq_per_wave, t, wave, lane,
+                               a.rule);
   if constexpr (HASH) {
     __shared__ u64 wsum[4];
     acc = wave_sum_u64(acc);

@@ -129,6 +129,39 @@ int fs3d_serve_specs(int32_t n, uint64_t seed, int32_t lo, int32_t hi, int32_t t
                      int32_t *ticks, double *arrival);
 int fs3d_threshold(uint64_t num, uint64_t den, uint64_t *thr);
 
+/* ---- batched worlds -------------------------------------------------------- */
+/* Many independent worlds of uneven sizes in one device arena.  No reference
+ * counterpart: the reference runs one scene in one process (src/main.cpp:6-16).
+ * FS3D_BATCH_PERSISTENT steps every world for nticks ticks in one persistent
+ * kernel launch (captured in a HIP graph): workgroups pull worlds from an atomic
+ * queue, largest first, and run all ticks of a world in place.
+ * FS3D_BATCH_PLAIN launches the single-world tick kernel once per world per
+ * tick (the baseline).  Both give bit-identical grids.  Worlds share the tick
+ * counter; nx <= 2048. */
+typedef struct fs3d_batch fs3d_batch; /* opaque */
+enum { FS3D_BATCH_PERSISTENT = 0, FS3D_BATCH_PLAIN = 1 };
+int fs3d_batch_create(int32_t n, const int32_t *dims /* n*3: nx,ny,nz */, fs3d_batch **out);
+int fs3d_batch_destroy(fs3d_batch *b); /* NULL is a no-op */
+int fs3d_batch_size(const fs3d_batch *b, int32_t *n);
+int fs3d_batch_upload(fs3d_batch *b, int32_t i, const uint8_t *host, size_t nbytes);
+int fs3d_batch_download(fs3d_batch *b, int32_t i, uint8_t *host, size_t nbytes);
+/* Rain fill of every world i with seeds[i], thr[i] (as fs3d_world_fill_rain). */
+int fs3d_batch_fill_rain(fs3d_batch *b, const uint64_t *seeds, const uint64_t *thr);
+int fs3d_batch_get_tick(const fs3d_batch *b, uint64_t *tick);
+int fs3d_batch_set_tick(fs3d_batch *b, uint64_t tick);
+/* Persistent-kernel shape: bytes per lane (8, 16, 32) and workgroup size (256,
+ * 512, 1024).  *grid receives the number of resident workgroups. */
+int fs3d_batch_set_config(fs3d_batch *b, int32_t seg, int32_t threads);
+int fs3d_batch_get_config(const fs3d_batch *b, int32_t *seg, int32_t *threads, int32_t *grid);
+/* Advances every world by nticks; when hashes != NULL writes each world's state
+ * hash after the last tick (hashes[n]).  Synchronises. */
+int fs3d_batch_run(fs3d_batch *b, int32_t nticks, int32_t mode, uint64_t *hashes);
+/* Device time (ms, HIP events) of the last fs3d_batch_run. */
+int fs3d_batch_last_run_ms(const fs3d_batch *b, double *ms);
+/* Times every persistent-kernel shape on a scratch copy of the arena for
+ * trial_ticks ticks (the batch itself does not advance) and keeps the fastest. */
+int fs3d_batch_autotune(fs3d_batch *b, int32_t trial_ticks, int32_t *seg, int32_t *threads, double *best_ms);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Average device time (ms) of the tick kernels launched by the last
  * fs3d_world_run on this world, measured with HIP events on the world's stream

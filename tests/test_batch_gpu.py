@@ -1,0 +1,85 @@
+"""Batched worlds on the GPU: the persistent kernel, the plain per-world loop and
+the oracle must agree bit for bit on every world."""
+import numpy as np
+import pytest
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def random_worlds(n, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(lo, hi + 1, size=(n, 3)).astype(np.int32)
+    grids = []
+    for nx, ny, nz in dims:
+        g = rng.choice(np.array([0, 0, 0, 1, 2, 2, 3, 3], np.uint8), size=(nz, ny, nx))
+        g[rng.random(g.shape) < 0.02] = 201
+        grids.append(g)
+    return dims, grids
+
+
+@pytest.mark.parametrize("config", [(8, 256), (16, 256), (16, 512), (16, 1024), (32, 256), (8, 1024)])
+def test_persistent_matches_oracle(gpu, O, config):
+    dims, grids = random_worlds(40, 11, 1, 70)
+    announce(f"batch persistent config={config} 40 random worlds, 7 ticks from t0=5")
+    with gpu.Batch(dims) as b:
+        b.set_config(*config)
+        for i, g in enumerate(grids):
+            b.upload(i, g)
+        b.tick_count = 5
+        h = b.run(7)
+        for i, g in enumerate(grids):
+            want, _ = O.run(g, 5, 7)
+            got = b.download(i)
+            assert (got == want).all(), (config, i, tuple(dims[i]))
+            assert int(h[i]) == O.state_hash(want)
+        assert b.tick_count == 12
+
+
+def test_plain_loop_matches_persistent(gpu, O):
+    dims, thr, seeds = gpu.batch_specs(300, 21, 16, 128)
+    announce("batch 300 spec worlds: 30 ticks persistent vs plain")
+    with gpu.Batch(dims) as a, gpu.Batch(dims) as c:
+        a.fill_rain(seeds, thr)
+        c.fill_rain(seeds, thr)
+        for i in (0, 7, 123):
+            g = O.scene_rain(tuple(int(v) for v in dims[i]), int(seeds[i]), int(thr[i]))
+            assert (a.download(i) == g).all()
+        h0 = a.hashes()
+        assert (h0 == c.hashes()).all()
+        ha = a.run(30)
+        hc = c.run(30, plain=True)
+        assert (ha == hc).all()
+        for i in (0, 7, 123, 299):
+            g = O.scene_rain(tuple(int(v) for v in dims[i]), int(seeds[i]), int(thr[i]))
+            want, _ = O.run(g, 0, 30, use_lut=True)
+            assert int(ha[i]) == O.state_hash(want)
+        # a second run continues from tick 30 in both modes
+        assert (a.run(11) == c.run(11, plain=True)).all()
+
+
+def test_autotune_keeps_state(gpu, tmp_path):
+    dims, thr, seeds = gpu.batch_specs(128, 5, 16, 96)
+    announce("batch autotune on 128 worlds")
+    with gpu.Batch(dims) as b:
+        b.fill_rain(seeds, thr)
+        h0 = b.hashes()
+        r = b.autotune(trial_ticks=4, cache_file=str(tmp_path / "t.json"))
+        assert r["seg"] in (8, 16, 32) and r["threads"] in (256, 512, 1024)
+        assert (b.hashes() == h0).all() and b.tick_count == 0
+        r2 = b.autotune(trial_ticks=4, cache_file=str(tmp_path / "t.json"))
+        assert r2["cached"] and (r2["seg"], r2["threads"]) == (r["seg"], r["threads"])
+        with gpu.Batch(dims) as c:
+            c.fill_rain(seeds, thr)
+            assert (b.run(9) == c.run(9, plain=True)).all()
+
+
+def test_batch_errors(gpu):
+    with pytest.raises(gpu.Fs3dError):
+        gpu.Batch([[0, 4, 4]])
+    with gpu.Batch([[4, 4, 4], [3, 5, 7]]) as b:
+        with pytest.raises(gpu.Fs3dError):
+            b.upload(1, np.zeros((4, 4, 4), np.uint8))
+        with pytest.raises(gpu.Fs3dError):
+            b.set_config(24, 256)
