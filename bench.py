@@ -89,7 +89,7 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
     with F.Batch(dims) as b:
         b.fill_rain(seeds, thr)
         tune = b.autotune(trial_ticks=10)
-        log(f"[bench] batched: autotune -> seg {tune['seg']} threads {tune['threads']}")
+        log(f"[bench] batched: autotune -> {tune}")
         h_sample = b.run(plain_ticks)
         with F.Batch(dims) as p:
             p.fill_rain(seeds, thr)
@@ -116,7 +116,8 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
                     gvox_per_s=vox * ticks / wall / 1e9, plain_ms_per_tick=plain_per_tick * 1e3,
                     plain_gvox_per_s=vox / plain_per_tick / 1e9,
                     speedup=(plain_per_tick * ticks) / wall, bit_exact_sample=exact,
-                    tuned={"seg": tune["seg"], "threads": tune["threads"]}, hash_digest=digest)
+                    tuned={"mode": {0: "persistent", 2: "wide"}.get(tune["mode"], tune["mode"]), "seg": tune["seg"],
+                           "threads": tune["threads"], "groups": tune["groups"]}, hash_digest=digest)
 
 
 def cpu_baseline(seconds=12.0):

@@ -17,7 +17,8 @@ import numpy as np
 
 from ._lib import Fs3dError, c_double, c_i32, c_u64, check, lib, ptr, vp
 
-PERSISTENT, PLAIN = 0, 1
+PERSISTENT, PLAIN, WIDE, TUNED = 0, 1, 2, 3
+MODE_NAMES = {PERSISTENT: "persistent", PLAIN: "plain", WIDE: "wide"}
 
 DEFAULT_TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
@@ -98,18 +99,25 @@ class Batch:
 
     @property
     def config(self):
-        s, t, g = c_i32(0), c_i32(0), c_i32(0)
-        check(lib().fs3d_batch_get_config(self.handle, ctypes.byref(s), ctypes.byref(t), ctypes.byref(g)))
-        return s.value, t.value, g.value
+        """(segment bytes, workgroup threads, row-quad groups, resident workgroups)."""
+        s, t, r, g = c_i32(0), c_i32(0), c_i32(0), c_i32(0)
+        check(lib().fs3d_batch_get_config(self.handle, ctypes.byref(s), ctypes.byref(t), ctypes.byref(r),
+                                          ctypes.byref(g)))
+        return s.value, t.value, r.value, g.value
 
-    def set_config(self, seg, threads):
-        check(lib().fs3d_batch_set_config(self.handle, seg, threads))
+    def set_config(self, seg, threads, groups=1):
+        check(lib().fs3d_batch_set_config(self.handle, seg, threads, groups))
 
-    def run(self, nticks, plain=False, hashes=True):
-        """Advances every world; returns the per-world hashes after the last tick."""
+    def run(self, nticks, plain=False, hashes=True, mode=None):
+        """Advances every world; returns the per-world hashes after the last tick.
+        mode: PERSISTENT, WIDE, PLAIN or TUNED (default: TUNED, or PLAIN when
+        plain=True)."""
+        if mode is None:
+            mode = PLAIN if plain else TUNED
+        if mode == TUNED and getattr(self, "_tuned_mode", None) is not None:
+            mode = self._tuned_mode
         out = np.zeros(self.n, np.uint64)
-        check(lib().fs3d_batch_run(self.handle, nticks, PLAIN if plain else PERSISTENT,
-                                   ptr(out, c_u64) if hashes else None))
+        check(lib().fs3d_batch_run(self.handle, nticks, mode, ptr(out, c_u64) if hashes else None))
         return out if hashes else None
 
     def hashes(self):
@@ -131,18 +139,22 @@ class Batch:
                     cache = json.load(f)
             except (OSError, ValueError):
                 cache = {}
-        if key in cache:
-            seg, thr = cache[key]["seg"], cache[key]["threads"]
-            self.set_config(seg, thr)
-            return dict(seg=seg, threads=thr, cached=True, ms=cache[key].get("ms"))
-        s, t, ms = c_i32(0), c_i32(0), c_double(0)
+        if key in cache and "mode" in cache[key]:
+            c = cache[key]
+            self.set_config(c["seg"], c["threads"], c["groups"])
+            self._tuned_mode = c["mode"]
+            return dict(seg=c["seg"], threads=c["threads"], groups=c["groups"], mode=c["mode"], cached=True,
+                        ms=c.get("ms"))
+        s, t, r, m, ms = c_i32(0), c_i32(0), c_i32(0), c_i32(0), c_double(0)
         check(lib().fs3d_batch_autotune(self.handle, trial_ticks, ctypes.byref(s), ctypes.byref(t),
-                                        ctypes.byref(ms)))
-        cache[key] = {"seg": s.value, "threads": t.value, "ms": ms.value, "trial_ticks": trial_ticks}
+                                        ctypes.byref(r), ctypes.byref(m), ctypes.byref(ms)))
+        self._tuned_mode = m.value
+        cache[key] = {"seg": s.value, "threads": t.value, "groups": r.value, "mode": m.value, "ms": ms.value,
+                      "trial_ticks": trial_ticks}
         if cache_file:
             try:
                 with open(cache_file, "w") as f:
                     json.dump(cache, f, indent=1, sort_keys=True)
             except OSError:
                 pass
-        return dict(seg=s.value, threads=t.value, cached=False, ms=ms.value)
+        return dict(seg=s.value, threads=t.value, groups=r.value, mode=m.value, cached=False, ms=ms.value)

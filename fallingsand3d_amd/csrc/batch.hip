@@ -22,7 +22,7 @@ namespace {
 
 using namespace tickcore;
 
-template <int S, int NT>
+template <int S, int NT, int NR>
 __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *order, int n, int nticks, u32 *queue,
                                                    const u64 *__restrict__ rule, u64 *hashes, int want_hash) {
   constexpr int NW = NT / 64;
@@ -48,12 +48,13 @@ __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *or
       const int p = (int)(t & 1u);
       const long long nrq = (long long)((w.ny + p + 1) >> 1) * ((w.nz + p + 1) >> 1);
       const long long nwav = (nrq + w.rqpw - 1) / w.rqpw;
+      // each wave takes NR row-quad groups per step (wv, wv + NW, ...)
       if (want_hash && k == nticks - 1) {
-        for (long long wv = wid; wv < nwav; wv += NW)
-          acc += tick_wave<S, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, lane, rule);
+        for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
+          acc += tick_waves<S, true, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule);
       } else {
-        for (long long wv = wid; wv < nwav; wv += NW)
-          tick_wave<S, false>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, lane, rule);
+        for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
+          tick_waves<S, false, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule);
       }
       __syncthreads();
     }
@@ -118,53 +119,69 @@ __global__ void reset_u32_kernel(u32 *p) {
   if (threadIdx.x == 0) *p = 0u;
 }
 
-template <int S, int NT>
+template <int S, int NT, int NR>
 hipError_t launch_batch_t(BWorld *worlds, const int *order, int n, int nticks, u32 *queue, const u64 *rule,
                           u64 *hashes, int want_hash, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<S, NT>), dim3(grid), dim3(NT), 0, s, worlds, order, n, nticks, queue, rule,
+  hipLaunchKernelGGL((batch_kernel<S, NT, NR>), dim3(grid), dim3(NT), 0, s, worlds, order, n, nticks, queue, rule,
                      hashes, want_hash);
   return hipGetLastError();
 }
 
-template <int S, int NT>
+template <int S, int NT, int NR>
 int occupancy_t() {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<S, NT>, NT, 0) != hipSuccess) return 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<S, NT, NR>, NT, 0) != hipSuccess) return 1;
   return nb > 0 ? nb : 1;
 }
 
 }  // namespace
 
-bool batch_config_ok(int seg, int threads) {
-  return (seg == 8 || seg == 16 || seg == 32) && (threads == 256 || threads == 512 || threads == 1024);
+// Supported persistent-kernel shapes: segment bytes x workgroup threads x
+// row-quad groups per wave step.
+bool batch_config_ok(int seg, int threads, int groups) {
+  const bool t_ok = threads == 256 || threads == 512 || threads == 1024;
+  const bool g_ok = groups == 1 || groups == 2 || groups == 4;
+  if (seg == 8 || seg == 16) return t_ok && g_ok;
+  return seg == 32 && threads == 256 && groups == 1;
 }
 
-#define FS3D_BATCH_DISPATCH(SEG, NT, CALL)                 \
-  switch (SEG * 10000 + NT) {                              \
-    case 80256: CALL(8, 256);                              \
-    case 80512: CALL(8, 512);                              \
-    case 81024: CALL(8, 1024);                             \
-    case 160256: CALL(16, 256);                            \
-    case 160512: CALL(16, 512);                            \
-    case 161024: CALL(16, 1024);                           \
-    case 320256: CALL(32, 256);                            \
-    case 320512: CALL(32, 512);                            \
-    default: CALL(32, 1024);                               \
+#define FS3D_BATCH_DISPATCH(SEG, NT, NR, CALL)     \
+  switch ((SEG) * 100000 + (NT) * 10 + (NR)) {      \
+    case 802561: CALL(8, 256, 1);                   \
+    case 802562: CALL(8, 256, 2);                   \
+    case 802564: CALL(8, 256, 4);                   \
+    case 805121: CALL(8, 512, 1);                   \
+    case 805122: CALL(8, 512, 2);                   \
+    case 805124: CALL(8, 512, 4);                   \
+    case 810241: CALL(8, 1024, 1);                  \
+    case 810242: CALL(8, 1024, 2);                  \
+    case 810244: CALL(8, 1024, 4);                  \
+    case 1602561: CALL(16, 256, 1);                 \
+    case 1602562: CALL(16, 256, 2);                 \
+    case 1602564: CALL(16, 256, 4);                 \
+    case 1605121: CALL(16, 512, 1);                 \
+    case 1605122: CALL(16, 512, 2);                 \
+    case 1605124: CALL(16, 512, 4);                 \
+    case 1610241: CALL(16, 1024, 1);                \
+    case 1610242: CALL(16, 1024, 2);                \
+    case 1610244: CALL(16, 1024, 4);                \
+    default: CALL(32, 256, 1);                      \
   }
 
-int batch_occupancy(int seg, int threads) {
-#define OCC(S_, N_) return occupancy_t<S_, N_>()
-  FS3D_BATCH_DISPATCH(seg, threads, OCC)
+int batch_occupancy(int seg, int threads, int groups) {
+#define OCC(S_, N_, R_) return occupancy_t<S_, N_, R_>()
+  FS3D_BATCH_DISPATCH(seg, threads, groups, OCC)
 #undef OCC
 }
 
-hipError_t launch_batch(int seg, int threads, BWorld *worlds, const int *order, int n, int nticks, u32 *queue,
-                        const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s) {
+hipError_t launch_batch(int seg, int threads, int groups, BWorld *worlds, const int *order, int n, int nticks,
+                        u32 *queue, const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s) {
   hipLaunchKernelGGL(reset_u32_kernel, dim3(1), dim3(64), 0, s, queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-#define LB(S_, N_) return launch_batch_t<S_, N_>(worlds, order, n, nticks, queue, rule, hashes, want_hash, grid, s)
-  FS3D_BATCH_DISPATCH(seg, threads, LB)
+#define LB(S_, N_, R_) \
+  return launch_batch_t<S_, N_, R_>(worlds, order, n, nticks, queue, rule, hashes, want_hash, grid, s)
+  FS3D_BATCH_DISPATCH(seg, threads, groups, LB)
 #undef LB
 }
 

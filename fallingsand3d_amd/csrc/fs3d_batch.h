@@ -1,4 +1,4 @@
-// Internal declarations of the batched-world kernels.
+// Internal declarations of the batched-world, wide multi-world and serving kernels.
 #pragma once
 
 #include "fs3d_internal.h"
@@ -9,17 +9,34 @@ namespace fs3d {
 struct BWorld {
   u8 *g;
   int nx, ny, nz;
-  int lr, rqpw;  // lanes per row / row quads per wave for the batch's segment width
+  int lr, rqpw;  // lanes per row / row quads per wave for the launch's segment width
+  int nt;        // wide launches: ticks this entry runs in the round (final tick hashes)
+  int slot;      // wide launches: index of the entry's hash slot
   int pad;
-  u64 tick;      // tick counter before the next run
+  u64 tick;      // tick counter before the next run (round)
 };
 
-bool batch_config_ok(int seg, int threads);
-int batch_occupancy(int seg, int threads);
-hipError_t launch_batch(int seg, int threads, BWorld *worlds, const int *order, int n, int nticks, u32 *queue,
-                        const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s);
+bool batch_config_ok(int seg, int threads, int groups);
+int batch_occupancy(int seg, int threads, int groups);
+hipError_t launch_batch(int seg, int threads, int groups, BWorld *worlds, const int *order, int n, int nticks,
+                        u32 *queue, const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s);
 hipError_t launch_batch_hash(const BWorld *worlds, int n, u64 *hashes, hipStream_t s);
 hipError_t launch_batch_fill_rain(const BWorld *worlds, int n, const u64 *seeds, const u64 *thr, size_t max_cells,
                                   hipStream_t s);
+
+// Waves of rqpw row quads each that a tick of parity p needs.
+__host__ __device__ inline long long world_waves_dev(int ny, int nz, int rqpw, int p) {
+  const long long nrq = (long long)((ny + p + 1) >> 1) * ((nz + p + 1) >> 1);
+  return (nrq + rqpw - 1) / rqpw;
+}
+
+// Wide launch k of a round: entry i owns workgroups [bpre[i], bpre[i+1]) (4 waves
+// each; none when it sits the launch out); blk2ent maps workgroup -> entry.
+hipError_t launch_wide(int seg, const BWorld *ents, const long long *bpre, const int *blk2ent, long long blocks, int k,
+                       const u64 *rule, u64 *hashes, hipStream_t s);
+hipError_t launch_expand(const long long *bpre, int n, int *blk2ent, hipStream_t s);
+hipError_t launch_zero_slots(u64 *hashes, const BWorld *ents, int n, hipStream_t s);
+long long world_waves(const BWorld &w, int p);
+long long world_blocks(const BWorld &w, int p);
 
 }  // namespace fs3d

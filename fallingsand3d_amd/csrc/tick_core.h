@@ -133,51 +133,65 @@ __device__ __forceinline__ u32 pack_states(u32 c0, u32 c1, u32 c2, u32 c3) {
   return __builtin_amdgcn_perm(b, a, 0x06020400u);
 }
 
-// Processes `wave` (a wave index in row-quad units of rqpw) of one tick at tick
-// counter t.  Returns this lane's share of the hash of the written cells when
-// HASH (0 otherwise).  Every lane of the wave must call it (DPP exchange).
-template <int S, bool HASH>
-__device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny, int nz, int LR_, int rqpw,
-                                         u32 t, long long wave, int lane, const u64 *__restrict__ rule) {
-  constexpr int ND = S / 4;  // dwords per lane per row
-  const int p = (int)(t & 1u);
+// One lane's share of one row quad: its S-byte segment of the 4 rows (r = dy +
+// 2*dz), loaded, plus where it lives.  Out-of-grid rows read as STATIC.
+template <int S>
+struct RowQuad {
+  u32 own[4][S / 4];
+  size_t rowoff[4];
+  int valid;  // bit r: row r is inside the grid
+  int m;      // valid bytes of the segment
+  int seg, yq, zq;
+};
+
+// Loads row-quad group `wave` (in units of rqpw row quads) of a tick of parity p.
+template <int S>
+__device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int LR, int rqpw,
+                                        int p, long long wave, int lane) {
+  constexpr int ND = S / 4;
   const int nyq = (ny + p + 1) >> 1, nzq = (nz + p + 1) >> 1;
   const long long nrq = (long long)nyq * nzq;
-  const int LR = LR_;
   const int rql = lane / LR;
-  const int seg = lane - rql * LR;
+  R.seg = lane - rql * LR;
   const long long q = wave * rqpw + rql;
   const bool active = rql < rqpw && q < nrq;
-  const int yq = active ? (int)(q % nyq) : 0;
-  const int zq = active ? (int)(q / nyq) : 0;
-  const int y0 = 2 * yq - p, z0 = 2 * zq - p;
-  const int xs = seg * S;
-  const int m = min(S, nx - xs);  // valid bytes of this segment
-
-  // ---- load the 4 rows (r = dy + 2*dz); out-of-grid rows read as STATIC ----
-  u32 own[4][ND];
-  size_t rowoff[4];
-  bool rvalid[4];
+  R.yq = active ? (int)(q % nyq) : 0;
+  R.zq = active ? (int)(q / nyq) : 0;
+  const int y0 = 2 * R.yq - p, z0 = 2 * R.zq - p;
+  const int xs = R.seg * S;
+  R.m = min(S, nx - xs);
+  R.valid = 0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int y = y0 + (r & 1), z = z0 + (r >> 1);
-    rvalid[r] = active && y >= 0 && y < ny && z >= 0 && z < nz;
-    rowoff[r] = ((size_t)(rvalid[r] ? z : 0) * ny + (rvalid[r] ? y : 0)) * nx + xs;
+    const bool ok = active && y >= 0 && y < ny && z >= 0 && z < nz;
+    R.valid |= ok ? (1 << r) : 0;
+    R.rowoff[r] = ((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs;
 #pragma unroll
-    for (int j = 0; j < ND; j++) own[r][j] = kStatic4;
-    if (rvalid[r]) load_seg<S>(src + rowoff[r], own[r]);
-    if (m < S) {
+    for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
+    if (ok) load_seg<S>(src + R.rowoff[r], R.own[r]);
+    if (R.m < S) {
 #pragma unroll
-      for (int j = 0; j < ND; j++) own[r][j] = keep_bytes(own[r][j], m - 4 * j);
+      for (int j = 0; j < ND; j++) R.own[r][j] = keep_bytes(R.own[r][j], R.m - 4 * j);
     }
   }
+}
+
+// Evaluates the blocks of a loaded row quad at tick t and stores the new bytes.
+// Returns this lane's share of the hash of the written cells when HASH.  Every
+// lane of the wave must call it together (DPP exchange).
+template <int S, bool HASH>
+__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule) {
+  constexpr int ND = S / 4;  // dwords per lane per row
+  const int p = (int)(t & 1u);
+  const int seg = R.seg;
 
   // ---- reserved bytes: wave-uniform slow path ----
   u32 anyres = 0;
 #pragma unroll
   for (int r = 0; r < 4; r++)
 #pragma unroll
-    for (int j = 0; j < ND; j++) anyres |= own[r][j] & 0xFCFCFCFCu;
+    for (int j = 0; j < ND; j++) anyres |= R.own[r][j] & 0xFCFCFCFCu;
   const bool slow = __any(anyres != 0);
 
   // ---- frame: even parity = own bytes; odd parity = [left byte, own, right byte] ----
@@ -185,18 +199,18 @@ __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny,
   u32 fr[4][NF];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const u32 from_l = lane_from_left(own[r][ND - 1]);
-    const u32 from_r = lane_from_right(own[r][0]);
+    const u32 from_l = lane_from_left(R.own[r][ND - 1]);
+    const u32 from_r = lane_from_right(R.own[r][0]);
     if (p) {
       const u32 lb = seg == 0 ? 0x01000000u : (from_l & 0xFF000000u);
       const u32 rb = seg == LR - 1 ? 0x01u : (from_r & 0xFFu);
-      fr[r][0] = __builtin_amdgcn_alignbyte(own[r][0], lb, 3);
+      fr[r][0] = __builtin_amdgcn_alignbyte(R.own[r][0], lb, 3);
 #pragma unroll
-      for (int j = 1; j < ND; j++) fr[r][j] = __builtin_amdgcn_alignbyte(own[r][j], own[r][j - 1], 3);
-      fr[r][ND] = __builtin_amdgcn_alignbyte(rb, own[r][ND - 1], 3);
+      for (int j = 1; j < ND; j++) fr[r][j] = __builtin_amdgcn_alignbyte(R.own[r][j], R.own[r][j - 1], 3);
+      fr[r][ND] = __builtin_amdgcn_alignbyte(rb, R.own[r][ND - 1], 3);
     } else {
 #pragma unroll
-      for (int j = 0; j < ND; j++) fr[r][j] = own[r][j];
+      for (int j = 0; j < ND; j++) fr[r][j] = R.own[r][j];
       fr[r][ND] = kStatic4;
     }
   }
@@ -207,7 +221,7 @@ __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny,
   constexpr int NG = S >= 64 ? 3 : 2;
   const u32 bx0 = (u32)(seg * (S / 2));
   const u32 g0 = bx0 >> 4;
-  const u32 yzt = (u32)yq * 0x85EBCA77u + (u32)zq * 0xC2B2AE3Du + t * 0x27D4EB2Fu;
+  const u32 yzt = (u32)R.yq * 0x85EBCA77u + (u32)R.zq * 0xC2B2AE3Du + t * 0x27D4EB2Fu;
   u32 wg[NG];
 #pragma unroll
   for (int i = 0; i < NG; i++) wg[i] = fmix32((g0 + (u32)i) * 0x9E3779B1u + yzt);
@@ -257,30 +271,56 @@ __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny,
     for (int j = 0; j < ND; j++) {
       u32 o = p ? __builtin_amdgcn_alignbyte(out[r][j + 1], out[r][j], 1) : out[r][j];
       if (slow) {
-        const u32 rm = reserved_mask(own[r][j]);
-        o = (own[r][j] & rm) | (o & ~rm);
+        const u32 rm = reserved_mask(R.own[r][j]);
+        o = (R.own[r][j] & rm) | (o & ~rm);
       }
       res[j] = o;
     }
-    if (rvalid[r]) {
-      u8 *dp = dst + rowoff[r];
-      if (m == S) {
+    if (R.valid >> r & 1) {
+      u8 *dp = dst + R.rowoff[r];
+      if (R.m == S) {
         store_seg<S>(dp, res);
       } else {
 #pragma unroll
         for (int k = 0; k < S; k++)
-          if (k < m) dp[k] = (u8)(res[k >> 2] >> (8 * (k & 3)));
+          if (k < R.m) dp[k] = (u8)(res[k >> 2] >> (8 * (k & 3)));
       }
       if constexpr (HASH) {
 #pragma unroll
         for (int k = 0; k < S; k++) {
           const u32 v = (res[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-          if (k < m && v) acc += dmix64(((u64)(rowoff[r] + k) << 8) | v);
+          if (k < R.m && v) acc += dmix64(((u64)(R.rowoff[r] + k) << 8) | v);
         }
       }
     }
   }
+  return acc;
+}
 
+// Processes row-quad group `wave` of one tick at tick counter t (src may equal
+// dst).  Every lane of the wave must call it.
+template <int S, bool HASH>
+__device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny, int nz, int LR, int rqpw, u32 t,
+                                         long long wave, int lane, const u64 *__restrict__ rule) {
+  RowQuad<S> R;
+  rq_load<S>(R, src, nx, ny, nz, LR, rqpw, (int)(t & 1u), wave, lane);
+  return rq_update<S, HASH>(R, dst, LR, t, rule);
+}
+
+// Processes NR row-quad groups (wave0, wave0 + stride, ...) with every load
+// issued before the first update, so NR groups' memory latency overlaps.  Row
+// quads are disjoint, so loading ahead is safe also in place.  Groups past the
+// end of the tick are inactive (no loads or stores).
+template <int S, bool HASH, int NR>
+__device__ __forceinline__ u64 tick_waves(const u8 *src, u8 *dst, int nx, int ny, int nz, int LR, int rqpw, u32 t,
+                                          long long wave0, long long stride, int lane, const u64 *__restrict__ rule) {
+  RowQuad<S> R[NR];
+  const int p = (int)(t & 1u);
+#pragma unroll
+  for (int i = 0; i < NR; i++) rq_load<S>(R[i], src, nx, ny, nz, LR, rqpw, p, wave0 + i * stride, lane);
+  u64 acc = 0;
+#pragma unroll
+  for (int i = 0; i < NR; i++) acc += rq_update<S, HASH>(R[i], dst, LR, t, rule);
   return acc;
 }
 

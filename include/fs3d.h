@@ -139,7 +139,10 @@ int fs3d_threshold(uint64_t num, uint64_t den, uint64_t *thr);
  * tick (the baseline).  Both give bit-identical grids.  Worlds share the tick
  * counter; nx <= 2048. */
 typedef struct fs3d_batch fs3d_batch; /* opaque */
-enum { FS3D_BATCH_PERSISTENT = 0, FS3D_BATCH_PLAIN = 1 };
+/* FS3D_BATCH_WIDE launches one wide multi-world tick per tick (all worlds'
+ * row quads over the whole GPU, captured in a graph); FS3D_BATCH_TUNED runs
+ * the mode the last autotune picked (persistent until then). */
+enum { FS3D_BATCH_PERSISTENT = 0, FS3D_BATCH_PLAIN = 1, FS3D_BATCH_WIDE = 2, FS3D_BATCH_TUNED = 3 };
 int fs3d_batch_create(int32_t n, const int32_t *dims /* n*3: nx,ny,nz */, fs3d_batch **out);
 int fs3d_batch_destroy(fs3d_batch *b); /* NULL is a no-op */
 int fs3d_batch_size(const fs3d_batch *b, int32_t *n);
@@ -149,18 +152,22 @@ int fs3d_batch_download(fs3d_batch *b, int32_t i, uint8_t *host, size_t nbytes);
 int fs3d_batch_fill_rain(fs3d_batch *b, const uint64_t *seeds, const uint64_t *thr);
 int fs3d_batch_get_tick(const fs3d_batch *b, uint64_t *tick);
 int fs3d_batch_set_tick(fs3d_batch *b, uint64_t tick);
-/* Persistent-kernel shape: bytes per lane (8, 16, 32) and workgroup size (256,
- * 512, 1024).  *grid receives the number of resident workgroups. */
-int fs3d_batch_set_config(fs3d_batch *b, int32_t seg, int32_t threads);
-int fs3d_batch_get_config(const fs3d_batch *b, int32_t *seg, int32_t *threads, int32_t *grid);
+/* Persistent-kernel shape: bytes per lane (8 or 16; 32 with 256 threads and 1
+ * group), workgroup size (256, 512, 1024) and row-quad groups each wave loads
+ * before updating (1, 2, 4: memory-level parallelism).  *grid receives the
+ * number of resident workgroups. */
+int fs3d_batch_set_config(fs3d_batch *b, int32_t seg, int32_t threads, int32_t groups);
+int fs3d_batch_get_config(const fs3d_batch *b, int32_t *seg, int32_t *threads, int32_t *groups, int32_t *grid);
 /* Advances every world by nticks; when hashes != NULL writes each world's state
  * hash after the last tick (hashes[n]).  Synchronises. */
 int fs3d_batch_run(fs3d_batch *b, int32_t nticks, int32_t mode, uint64_t *hashes);
 /* Device time (ms, HIP events) of the last fs3d_batch_run. */
 int fs3d_batch_last_run_ms(const fs3d_batch *b, double *ms);
-/* Times every persistent-kernel shape on a scratch copy of the arena for
- * trial_ticks ticks (the batch itself does not advance) and keeps the fastest. */
-int fs3d_batch_autotune(fs3d_batch *b, int32_t trial_ticks, int32_t *seg, int32_t *threads, double *best_ms);
+/* Times every persistent-kernel shape and the wide mode at each segment width
+ * on a scratch copy of the arena for trial_ticks ticks (the batch itself does
+ * not advance), keeps the fastest and makes its mode the FS3D_BATCH_TUNED one. */
+int fs3d_batch_autotune(fs3d_batch *b, int32_t trial_ticks, int32_t *seg, int32_t *threads, int32_t *groups,
+                        int32_t *mode, double *best_ms);
 
 /* ---- timing ---------------------------------------------------------------- */
 /* Average device time (ms) of the tick kernels launched by the last

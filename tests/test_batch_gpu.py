@@ -19,7 +19,8 @@ def random_worlds(n, seed, lo, hi):
     return dims, grids
 
 
-@pytest.mark.parametrize("config", [(8, 256), (16, 256), (16, 512), (16, 1024), (32, 256), (8, 1024)])
+@pytest.mark.parametrize("config", [(8, 256, 1), (16, 256, 1), (16, 512, 2), (16, 1024, 4), (32, 256, 1),
+                                    (8, 1024, 1), (8, 256, 4), (8, 512, 2), (16, 256, 4)])
 def test_persistent_matches_oracle(gpu, O, config):
     dims, grids = random_worlds(40, 11, 1, 70)
     announce(f"batch persistent config={config} 40 random worlds, 7 ticks from t0=5")
@@ -35,6 +36,27 @@ def test_persistent_matches_oracle(gpu, O, config):
             assert (got == want).all(), (config, i, tuple(dims[i]))
             assert int(h[i]) == O.state_hash(want)
         assert b.tick_count == 12
+
+
+@pytest.mark.parametrize("seg", [8, 16, 32])
+def test_wide_matches_oracle(gpu, O, seg):
+    from fallingsand3d_amd.batch import WIDE
+    dims, grids = random_worlds(40, 12 + seg, 1, 70)
+    announce(f"batch wide seg={seg} 40 random worlds, 7 + 4 ticks from t0=3")
+    with gpu.Batch(dims) as b:
+        b.set_config(seg, 256, 1)
+        for i, g in enumerate(grids):
+            b.upload(i, g)
+        b.tick_count = 3
+        h = b.run(7, mode=WIDE)
+        h2 = b.run(4, mode=WIDE)  # starts on the other parity
+        for i, g in enumerate(grids):
+            want, _ = O.run(g, 3, 7)
+            assert int(h[i]) == O.state_hash(want), (seg, i)
+            want2, _ = O.run(want, 10, 4)
+            assert (b.download(i) == want2).all(), (seg, i, tuple(dims[i]))
+            assert int(h2[i]) == O.state_hash(want2)
+        assert b.tick_count == 14
 
 
 def test_plain_loop_matches_persistent(gpu, O):
@@ -66,10 +88,10 @@ def test_autotune_keeps_state(gpu, tmp_path):
         b.fill_rain(seeds, thr)
         h0 = b.hashes()
         r = b.autotune(trial_ticks=4, cache_file=str(tmp_path / "t.json"))
-        assert r["seg"] in (8, 16, 32) and r["threads"] in (256, 512, 1024)
+        assert r["seg"] in (8, 16, 32) and r["threads"] in (256, 512, 1024) and r["groups"] in (1, 2, 4)
         assert (b.hashes() == h0).all() and b.tick_count == 0
         r2 = b.autotune(trial_ticks=4, cache_file=str(tmp_path / "t.json"))
-        assert r2["cached"] and (r2["seg"], r2["threads"]) == (r["seg"], r["threads"])
+        assert r2["cached"] and (r2["seg"], r2["threads"], r2["groups"]) == (r["seg"], r["threads"], r["groups"])
         with gpu.Batch(dims) as c:
             c.fill_rain(seeds, thr)
             assert (b.run(9) == c.run(9, plain=True)).all()
@@ -82,4 +104,4 @@ def test_batch_errors(gpu):
         with pytest.raises(gpu.Fs3dError):
             b.upload(1, np.zeros((4, 4, 4), np.uint8))
         with pytest.raises(gpu.Fs3dError):
-            b.set_config(24, 256)
+            b.set_config(24, 256, 1)
