@@ -120,6 +120,33 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
                            "threads": tune["threads"], "groups": tune["groups"]}, hash_digest=digest)
 
 
+def bench_serving(F, seconds=3.0, lam=2000.0, seed=5, round_ticks=8, check=8):
+    """Poisson stream of requests at `lam` per second for `seconds` seconds of
+    arrivals (edges log-uniform 32-256, ticks uniform 10-500, rain fill with
+    density U(0.05, 0.5)) through the continuous-batching server under a
+    200 GiB budget, replayed in real time; `check` requests spread over the
+    stream are re-run alone and their final hashes compared."""
+    n = int(seconds * lam)
+    dims, thr, seeds, ticks, arr = F.serve_specs(n, seed, 32, 256, 10, 500, lam)
+    with F.Server(budget_bytes=200 << 30, round_ticks=round_ticks, max_active=1 << 16) as s:
+        t0 = time.perf_counter()
+        ids = s.replay(dims, seeds, thr, ticks, arr)
+        wall = time.perf_counter() - t0
+        st = s.stats()
+        res = [s.result(int(i)) for i in ids]
+    exact = True
+    for i in range(0, n, max(1, n // check)):
+        with F.World(tuple(int(v) for v in dims[i])) as w:
+            w.fill_rain(int(seeds[i]), int(thr[i]))
+            w.step(int(ticks[i]))
+            exact = exact and (w.hash() == res[i][1]) and res[i][0] == 2
+    log(f"[bench] serving: {n} requests in {wall:.2f}s, tick p50 {st['tick_p50_ms']:.3f} ms p99 "
+        f"{st['tick_p99_ms']:.3f} ms, request p50 {st['req_p50_ms']:.1f} ms p99 {st['req_p99_ms']:.1f} ms, "
+        f"bit-exact={exact}")
+    return dict(n=n, wall=wall, stats=st, exact=exact,
+                gvox_per_s=st["voxel_updates"] / wall / 1e9, busy_frac=st["busy_ms"] / (wall * 1e3))
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's tick (table-driven, one thread) on the 512^3 box scene, timed
     for about `seconds` of host work."""
@@ -142,7 +169,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched"], default=None)
+    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving"], default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -166,6 +193,11 @@ def main():
     if args.only == "batched":
         bb = bench_batched(F, ticks=args.steps)
         print(json.dumps({"workload": "batched", **bb}), flush=True)
+        return
+    if args.only == "serving":
+        sv = bench_serving(F)
+        print(json.dumps({"workload": "serving", **{k: v for k, v in sv.items() if k != "stats"},
+                          **sv["stats"]}), flush=True)
         return
 
     r = bench_boxes512(F, args.steps, args.warmup)
@@ -214,6 +246,16 @@ def main():
             "plain_ms_per_tick_sampled": round(bb["plain_ms_per_tick"], 3),
             "speedup_vs_plain": round(bb["speedup"], 2), "bit_exact_sample": bb["bit_exact_sample"],
             "autotuned": bb["tuned"], "final_hash_digest": bb["hash_digest"]}
+        sv = bench_serving(F)
+        st = sv["stats"]
+        extra["serving"] = {
+            "workload": "Poisson 2000 req/s for 3 s, edges log-uniform 32-256, ticks U{10..500}, rain fill, "
+                        "200 GiB budget, rounds of 8 ticks",
+            "requests": sv["n"], "wall_s": round(sv["wall"], 3), "gvox_per_s": round(sv["gvox_per_s"], 2),
+            "tick_p50_ms": round(st["tick_p50_ms"], 4), "tick_p99_ms": round(st["tick_p99_ms"], 4),
+            "request_p50_ms": round(st["req_p50_ms"], 2), "request_p99_ms": round(st["req_p99_ms"], 2),
+            "device_busy_frac": round(sv["busy_frac"], 3), "peak_active": st["peak_active"],
+            "peak_bytes": st["peak_bytes"], "rejected": st["rejected"], "bit_exact_sample": sv["exact"]}
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

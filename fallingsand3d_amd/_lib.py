@@ -45,6 +45,19 @@ class PillarParams(ctypes.Structure):
                  "pool_d_min", "pool_d_max", "pool_h_min", "pool_h_max")]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", c_i64), ("round_ticks", c_i32), ("max_active", c_i32), ("seg", c_i32),
+                ("reserved", c_i32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_i64), ("completed", c_i64), ("rejected", c_i64), ("rounds", c_i64),
+                ("launches", c_i64), ("tick_p50_ms", c_double), ("tick_p99_ms", c_double),
+                ("tick_max_ms", c_double), ("req_p50_ms", c_double), ("req_p99_ms", c_double),
+                ("voxel_updates", c_double), ("busy_ms", c_double), ("peak_bytes", c_i64), ("used_bytes", c_i64),
+                ("peak_active", c_i32), ("reserved", c_i32)]
+
+
 # name -> (restype, argtypes); every entry point of include/fs3d.h
 SIGNATURES = {
     "fs3d_last_error": (ctypes.c_char_p, []),
@@ -92,6 +105,14 @@ SIGNATURES = {
     "fs3d_batch_run": (c_int, [vp, c_i32, c_i32, P(c_u64)]),
     "fs3d_batch_last_run_ms": (c_int, [vp, P(c_double)]),
     "fs3d_batch_autotune": (c_int, [vp, c_i32, P(c_i32), P(c_i32), P(c_i32), P(c_i32), P(c_double)]),
+    "fs3d_server_create": (c_int, [P(ServerConfig), P(vp)]),
+    "fs3d_server_destroy": (c_int, [vp]),
+    "fs3d_server_submit": (c_int, [vp, P(c_i32), c_u64, c_u64, c_i32, P(c_i64)]),
+    "fs3d_server_poll": (c_int, [vp, P(c_i32), P(c_i32), P(c_i64)]),
+    "fs3d_server_drain": (c_int, [vp]),
+    "fs3d_server_result": (c_int, [vp, c_i64, P(c_i32), P(c_u64), P(c_double)]),
+    "fs3d_server_replay": (c_int, [vp, c_i32, P(c_i32), P(c_u64), P(c_u64), P(c_i32), P(c_double), P(c_i64)]),
+    "fs3d_server_get_stats": (c_int, [vp, P(ServerStats)]),
 }
 
 _LIB = None

@@ -69,7 +69,7 @@ int sync_table(fs3d_batch *b, int nt = 0) {
     w.rqpw = 64 / w.lr;
     w.nt = nt;
     w.slot = i;
-    w.pad = 0;
+    w.fin = 1;  // a batch run's last tick is the one hashed
     w.tick = b->tick;
   }
   FS3D_HIP(hipMemcpyAsync(b->d_worlds, b->h_worlds.data(), sizeof(BWorld) * b->n, hipMemcpyHostToDevice, b->stream));

@@ -10,9 +10,9 @@ struct BWorld {
   u8 *g;
   int nx, ny, nz;
   int lr, rqpw;  // lanes per row / row quads per wave for the launch's segment width
-  int nt;        // wide launches: ticks this entry runs in the round (final tick hashes)
+  int nt;        // wide launches: ticks this entry runs in the round
   int slot;      // wide launches: index of the entry's hash slot
-  int pad;
+  int fin;       // wide launches: 1 when the round's last tick is the entry's final one (hash it)
   u64 tick;      // tick counter before the next run (round)
 };
 

@@ -7,8 +7,8 @@
 // has none) and blk2ent, the entry of every workgroup, expanded on the device
 // from the prefix.  Each workgroup reads its entry with one scalar load and its
 // waves run the ordinary row-quad tick in place.  An entry whose final tick
-// this is (k + 1 == entry.nt) hashes its new state; every wave adds its share
-// to hashes[entry.slot] with one atomic.
+// this is (k + 1 == entry.nt with entry.fin set) hashes its new state; every
+// wave adds its share to hashes[entry.slot] with one atomic.
 //
 // Used by the serving loop (continuous batching at tick granularity) and as the
 // batch's third mode.
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void wide_kernel(const BWorld *__restrict__ en
   const u32 t = (u32)(w.tick + (u64)k);
   if (wave >= world_waves_dev(w.ny, w.nz, w.rqpw, (int)(t & 1u))) return;  // wave-uniform; no barriers below
   const int lane = threadIdx.x & 63;
-  if (k + 1 == w.nt && hashes) {
+  if (k + 1 == w.nt && w.fin && hashes) {
     u64 acc = tick_wave<S, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule);
     acc = wave_sum_u64(acc);
     if (lane == 0 && acc) atomicAdd((unsigned long long *)(hashes + w.slot), (unsigned long long)acc);

@@ -169,6 +169,54 @@ int fs3d_batch_last_run_ms(const fs3d_batch *b, double *ms);
 int fs3d_batch_autotune(fs3d_batch *b, int32_t trial_ticks, int32_t *seg, int32_t *threads, int32_t *groups,
                         int32_t *mode, double *best_ms);
 
+/* ---- serving ----------------------------------------------------------------- */
+/* Continuous batching of simulation requests.  No reference counterpart (the
+ * reference is one interactive window, src/engine/engine.cpp:54-74).  Requests
+ * (dims, rain-fill seed and threshold, tick count) are admitted in arrival order
+ * while their grids fit budget_bytes (grids are rounded to size classes with at
+ * most 25% slack; a request larger than the whole budget is rejected), filled on
+ * the device, kept resident, and advanced in rounds of up to round_ticks ticks,
+ * each tick one wide launch over every active world; requests arriving meanwhile
+ * join at the next round.  A request's final hash is fused into its last tick.
+ * Calls on one server must be serialised. */
+typedef struct fs3d_server fs3d_server; /* opaque */
+enum { FS3D_REQ_QUEUED = 0, FS3D_REQ_ACTIVE = 1, FS3D_REQ_DONE = 2, FS3D_REQ_REJECTED = 3 };
+typedef struct fs3d_server_config {
+  int64_t budget_bytes; /* HBM budget for resident grids */
+  int32_t round_ticks;  /* ticks per round (>= 1) */
+  int32_t max_active;   /* concurrent requests (hash slots) */
+  int32_t seg;          /* tick segment bytes: 8, 16, 32 (0 = 8; nx <= 64 * seg) */
+  int32_t reserved;
+} fs3d_server_config;
+typedef struct fs3d_server_stats {
+  int64_t submitted, completed, rejected;
+  int64_t rounds, launches;
+  double tick_p50_ms, tick_p99_ms, tick_max_ms; /* per launch: one tick of every world in it */
+  double req_p50_ms, req_p99_ms;                /* submission -> final hash observed */
+  double voxel_updates;                         /* cell updates enqueued so far */
+  double busy_ms;                               /* summed device time of completed launches */
+  int64_t peak_bytes, used_bytes;               /* admitted grid bytes: high-water mark, now */
+  int32_t peak_active;
+  int32_t reserved;
+} fs3d_server_stats;
+int fs3d_server_create(const fs3d_server_config *cfg, fs3d_server **out);
+int fs3d_server_destroy(fs3d_server *s); /* NULL is a no-op; waits for work in flight */
+/* Queues a request (ticks >= 1); *id is its index, in submission order. */
+int fs3d_server_submit(fs3d_server *s, const int32_t dims[3], uint64_t seed, uint64_t thr, int32_t ticks,
+                       int64_t *id);
+/* One scheduling step: observe finished rounds, admit, plan and enqueue a round
+ * (at most two in flight).  Never blocks on the GPU. */
+int fs3d_server_poll(fs3d_server *s, int32_t *n_active, int32_t *n_queued, int64_t *n_done);
+/* Polls until every submitted request is done or rejected. */
+int fs3d_server_drain(fs3d_server *s);
+int fs3d_server_result(const fs3d_server *s, int64_t id, int32_t *status, uint64_t *hash, double *latency_ms);
+/* Real-time driver: submits request i when arrival[i] seconds have passed since
+ * the call began, polling in between, until all are done; ids[i] receives the
+ * request ids. */
+int fs3d_server_replay(fs3d_server *s, int32_t n, const int32_t *dims, const uint64_t *seeds, const uint64_t *thr,
+                       const int32_t *ticks, const double *arrival, int64_t *ids);
+int fs3d_server_get_stats(const fs3d_server *s, fs3d_server_stats *out);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Average device time (ms) of the tick kernels launched by the last
  * fs3d_world_run on this world, measured with HIP events on the world's stream
