@@ -9,3 +9,4 @@ from .world import (EMPTY, FLUID, SAND, STATIC, BOXES_512, PILLARS_2048, World, 
                     batch_specs, copy_probe, device_count, init, rule_lut, serve_specs, threshold)
 from .batch import Batch  # noqa: F401,E402
 from .serve import Server  # noqa: F401,E402
+from . import render  # noqa: F401,E402

@@ -45,6 +45,10 @@ class PillarParams(ctypes.Structure):
                  "pool_d_min", "pool_d_max", "pool_h_min", "pool_h_max")]
 
 
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", c_float * 3), ("basis", c_float * 9), ("light", c_float * 3), ("aspect", c_float)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", c_i64), ("round_ticks", c_i32), ("max_active", c_i32), ("seg", c_i32),
                 ("reserved", c_i32)]
@@ -113,6 +117,13 @@ SIGNATURES = {
     "fs3d_server_result": (c_int, [vp, c_i64, P(c_i32), P(c_u64), P(c_double)]),
     "fs3d_server_replay": (c_int, [vp, c_i32, P(c_i32), P(c_u64), P(c_u64), P(c_i32), P(c_double), P(c_i64)]),
     "fs3d_server_get_stats": (c_int, [vp, P(ServerStats)]),
+    "fs3d_render_sdf": (c_int, [P(c_float), c_float, c_i32, c_i32, P(c_float)]),
+    "fs3d_world_render": (c_int, [vp, P(Camera), c_i32, c_i32, P(c_float), P(c_u64), P(c_u8)]),
+    "fs3d_world_last_render_ms": (c_int, [vp, P(c_double)]),
+    "fs3d_orbit_camera": (c_int, [P(c_i32), c_double, c_double, c_double, c_float, P(Camera)]),
+    "fs3d_camera_globaldata": (c_int, [P(c_float), P(c_float), P(c_float)]),
+    "fs3d_camera_integrate": (c_int, [P(c_float), P(c_float), c_u32]),
+    "fs3d_palette": (c_int, [P(c_float)]),
 }
 
 _LIB = None

@@ -107,6 +107,9 @@ void release(fs3d_world *w) {
   for (int i = 0; i < 2; i++)
     if (w->buf[i]) (void)hipFree(w->buf[i]);
   if (w->d_parts) (void)hipFree(w->d_parts);
+  if (w->d_rgb) (void)hipFree(w->d_rgb);
+  if (w->d_hit) (void)hipFree(w->d_hit);
+  if (w->d_face) (void)hipFree(w->d_face);
   if (w->d_small) (void)hipFree(w->d_small);
   if (w->h_pinned) (void)hipHostFree(w->h_pinned);
   if (w->ev0) (void)hipEventDestroy(w->ev0);

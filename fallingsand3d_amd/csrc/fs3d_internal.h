@@ -106,6 +106,12 @@ struct fs3d_world {
   int seg_req = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_run_ms = 0.0;
+  // render outputs on the device (grown on demand)
+  float *d_rgb = nullptr;
+  fs3d::u64 *d_hit = nullptr;
+  fs3d::u8 *d_face = nullptr;
+  size_t render_px = 0;
+  double last_render_ms = 0.0;
   // cached graphs keyed by (nticks, hash_every, cur)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 };

@@ -1,0 +1,23 @@
+// Internal declarations of the render kernels.
+#pragma once
+
+#include "fs3d_internal.h"
+
+namespace fs3d {
+
+// Camera of the voxel ray caster: ray origin, basis (screen-left, screen-down,
+// forward; world dir = dx*b0 + dy*b1 + dz*b2 before normalising), light position
+// and the aspect ratio the pixel mapping divides uv.y by.
+struct RenderCam {
+  float origin[3];
+  float basis[9];
+  float light[3];
+  float aspect;
+};
+
+hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float *d_rgba, hipStream_t s);
+hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
+                      u64 *d_hit, u8 *d_face, hipStream_t s);
+void palette_host(float *rgba);
+
+}  // namespace fs3d

@@ -1,0 +1,230 @@
+// Renderers: the HIP port of the reference's full-screen sphere tracer and the
+// voxel-grid ray caster that hands a world's grid to the screen.
+//
+// Pixel -> ray follows the reference exactly (shaders/fs_raymarch.frag:67-75,
+// shaders/fs_raymarch.vert:30-36, quad UVs src/engine/rendering/renderer.cpp:1253-1267):
+// inUV = ((W - px - 0.5)/W, (py + 0.5)/H), py = 0 at the top; uv = inUV*2 - 1;
+// uv.y *= 1/aspect; dir = normalize(uv, 1) (camera +x = screen-left, +y =
+// screen-down, +z = forward).  normalize(v) = v / sqrt((x*x + y*y) + z*z) and
+// dot(a,b) = (ax*bx + ay*by) + az*bz, each float op a single correctly rounded
+// IEEE-754 operation (the library is built with -ffp-contract=off; HIP's fp32
+// division and sqrt are correctly rounded by default).
+//
+// sdf_kernel: shaders/fs_raymarch.frag:10-81 -- sphere of radius 0.5 at the
+// origin, <= 64 steps, hit below 0.001, stop once t > 1000 (checked after the
+// hit test), central-difference normal with h = 0.001, colour (1,0,0) *
+// max(0.05, dot(n, normalize(p - (2,5,3)))), miss = black, alpha 1.
+//
+// dda_kernel: one thread per pixel; world dir = dx*b0 + dy*b1 + dz*b2 from the
+// camera basis; slab test against [0,nx]x[0,ny]x[0,nz]; Amanatides & Woo voxel
+// traversal (ties x, then y, then z) to the first non-EMPTY cell; face = the one
+// entered through (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z, 6 = started inside);
+// colour = palette entry (STATIC 127, SAND 162, FLUID 137, other bytes their own
+// index) * max(0.05, dot(n, normalize(p - light))) -- fs_raymarch's lighting with
+// the face normal (-dir when inside).  Miss: black, hit = UINT64_MAX, face 255.
+#include "fs3d_internal.h"
+#include "fs3d_render.h"
+
+namespace fs3d {
+namespace {
+
+__constant__ u32 kPaletteBits[1024] = {
+#include "palette.inc"
+};
+
+struct F3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ F3 nrm3(float x, float y, float z) {
+  const float l = sqrtf((x * x + y * y) + z * z);
+  return F3{x / l, y / l, z / l};
+}
+
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+__device__ __forceinline__ F3 pixel_dir(int px, int py, int W, int H, float aspect) {
+  const float u = ((float)(W - px) - 0.5f) / (float)W;
+  const float v = ((float)py + 0.5f) / (float)H;
+  const float ux = u * 2.0f - 1.0f;
+  float uy = v * 2.0f - 1.0f;
+  const float inv = 1.0f / aspect;
+  uy = uy * inv;
+  return nrm3(ux, uy, 1.0f);
+}
+
+__device__ __forceinline__ float sdf_map(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z) - 0.5f; }
+
+__global__ __launch_bounds__(256) void sdf_kernel(float ox, float oy, float oz, float aspect, int W, int H,
+                                                  float *rgba) {
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= W || py >= H) return;
+  const F3 d = pixel_dir(px, py, W, H, aspect);
+  float r = 0.0f, gg = 0.0f, b = 0.0f;
+  float t = 0.0f;
+  for (int i = 0; i < 64; i++) {
+    const float p0 = ox + t * d.x, p1 = oy + t * d.y, p2 = oz + t * d.z;
+    const float dist = sdf_map(p0, p1, p2);
+    if (dist < 0.001f) {
+      const float h = 0.001f;
+      const float gx = sdf_map(p0 + h, p1 + 0.0f, p2 + 0.0f) - sdf_map(p0 - h, p1 - 0.0f, p2 - 0.0f);
+      const float gy = sdf_map(p0 + 0.0f, p1 + h, p2 + 0.0f) - sdf_map(p0 - 0.0f, p1 - h, p2 - 0.0f);
+      const float gz = sdf_map(p0 + 0.0f, p1 + 0.0f, p2 + h) - sdf_map(p0 - 0.0f, p1 - 0.0f, p2 - h);
+      const F3 n = nrm3(gx, gy, gz);
+      const F3 dl = nrm3(p0 - 2.0f, p1 - 5.0f, p2 - 3.0f);
+      const float diff = fmaxf(0.05f, dot3(n, dl));
+      r = 1.0f * diff;
+      gg = 0.0f * diff;
+      b = 0.0f * diff;
+      break;
+    } else if (t > 1000.0f) {
+      break;
+    }
+    t = t + dist;
+  }
+  float *out = rgba + ((size_t)py * W + px) * 4;
+  out[0] = r;
+  out[1] = gg;
+  out[2] = b;
+  out[3] = 1.0f;
+}
+
+__device__ __forceinline__ int palette_index(u32 m) {
+  return m == FS3D_STATIC ? 127 : m == FS3D_SAND ? 162 : m == FS3D_FLUID ? 137 : (int)m;
+}
+
+__global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, RenderCam cam,
+                                                  int W, int H, float *rgb, u64 *hit, u8 *face) {
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= W || py >= H) return;
+  const size_t pix = (size_t)py * W + px;
+  const F3 dc = pixel_dir(px, py, W, H, cam.aspect);
+  const float *bx = cam.basis, *by = cam.basis + 3, *bz = cam.basis + 6;
+  const F3 dd = nrm3((dc.x * bx[0] + dc.y * by[0]) + dc.z * bz[0], (dc.x * bx[1] + dc.y * by[1]) + dc.z * bz[1],
+                     (dc.x * bx[2] + dc.y * by[2]) + dc.z * bz[2]);
+  const float d[3] = {dd.x, dd.y, dd.z};
+  const float o[3] = {cam.origin[0], cam.origin[1], cam.origin[2]};
+  const float nd[3] = {(float)nx, (float)ny, (float)nz};
+  const int ni[3] = {nx, ny, nz};
+  float outc[3] = {0.0f, 0.0f, 0.0f};
+  u64 hout = ~0ull;
+  u8 fout = 255;
+
+  float tmin = 0.0f, tmax = __builtin_inff();
+  int enter = -1;
+  bool miss = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (d[a] == 0.0f) {
+      if (o[a] < 0.0f || o[a] > nd[a]) miss = true;
+      continue;
+    }
+    const float t1 = (0.0f - o[a]) / d[a];
+    const float t2 = (nd[a] - o[a]) / d[a];
+    const float lo = fminf(t1, t2), hi = fmaxf(t1, t2);
+    if (lo > tmin) {
+      tmin = lo;
+      enter = a;
+    }
+    if (hi < tmax) tmax = hi;
+  }
+  if (!miss && !(tmin > tmax)) {
+    int idx[3], step[3];
+    float tnext[3], tdelta[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float p = o[a] + tmin * d[a];
+      int i = (int)floorf(p);
+      i = i < 0 ? 0 : (i > ni[a] - 1 ? ni[a] - 1 : i);
+      idx[a] = i;
+      if (d[a] > 0.0f) {
+        step[a] = 1;
+        tdelta[a] = fabsf(1.0f / d[a]);
+        tnext[a] = ((float)(i + 1) - o[a]) / d[a];
+      } else if (d[a] < 0.0f) {
+        step[a] = -1;
+        tdelta[a] = fabsf(1.0f / d[a]);
+        tnext[a] = ((float)i - o[a]) / d[a];
+      } else {
+        step[a] = 0;
+        tdelta[a] = __builtin_inff();
+        tnext[a] = __builtin_inff();
+      }
+    }
+    int f = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
+    float t = tmin;
+    const int maxit = nx + ny + nz + 3;
+    for (int it = 0; it < maxit; it++) {
+      const size_t li = ((size_t)idx[2] * ny + idx[1]) * nx + idx[0];
+      const u32 m = g[li];
+      if (m != 0) {
+        const int ci = palette_index(m);
+        F3 n{0.0f, 0.0f, 0.0f};
+        if (f == 6) {
+          n = F3{-d[0], -d[1], -d[2]};
+        } else {
+          const float s = (f & 1) ? 1.0f : -1.0f;
+          n.x = (f >> 1) == 0 ? s : 0.0f;
+          n.y = (f >> 1) == 1 ? s : 0.0f;
+          n.z = (f >> 1) == 2 ? s : 0.0f;
+        }
+        const float h0 = o[0] + t * d[0], h1 = o[1] + t * d[1], h2 = o[2] + t * d[2];
+        const F3 dl = nrm3(h0 - cam.light[0], h1 - cam.light[1], h2 - cam.light[2]);
+        const float diff = fmaxf(0.05f, dot3(n, dl));
+        outc[0] = __uint_as_float(kPaletteBits[4 * ci + 0]) * diff;
+        outc[1] = __uint_as_float(kPaletteBits[4 * ci + 1]) * diff;
+        outc[2] = __uint_as_float(kPaletteBits[4 * ci + 2]) * diff;
+        hout = (u64)li;
+        fout = (u8)f;
+        break;
+      }
+      int a;
+      if (tnext[0] <= tnext[1] && tnext[0] <= tnext[2])
+        a = 0;
+      else if (tnext[1] <= tnext[2])
+        a = 1;
+      else
+        a = 2;
+      if (step[a] == 0) break;
+      t = tnext[a];
+      idx[a] += step[a];
+      if (idx[a] < 0 || idx[a] >= ni[a]) break;
+      tnext[a] = tnext[a] + tdelta[a];
+      f = 2 * a + (step[a] > 0 ? 0 : 1);
+    }
+  }
+  rgb[pix * 3 + 0] = outc[0];
+  rgb[pix * 3 + 1] = outc[1];
+  rgb[pix * 3 + 2] = outc[2];
+  hit[pix] = hout;
+  face[pix] = fout;
+}
+
+}  // namespace
+
+hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float *d_rgba, hipStream_t s) {
+  const dim3 grid((W + 15) / 16, (H + 15) / 16);
+  hipLaunchKernelGGL(sdf_kernel, grid, dim3(256), 0, s, cam_pos[0], cam_pos[1], cam_pos[2], aspect, W, H, d_rgba);
+  return hipGetLastError();
+}
+
+hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
+                      u64 *d_hit, u8 *d_face, hipStream_t s) {
+  const dim3 grid((W + 15) / 16, (H + 15) / 16);
+  hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face);
+  return hipGetLastError();
+}
+
+void palette_host(float *rgba) {
+  static const u32 bits[1024] = {
+#include "palette.inc"
+  };
+  for (int i = 0; i < 1024; i++) {
+    u32 b = bits[i];
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    rgba[i] = f;
+  }
+}
+
+}  // namespace fs3d

@@ -169,6 +169,49 @@ int fs3d_batch_last_run_ms(const fs3d_batch *b, double *ms);
 int fs3d_batch_autotune(fs3d_batch *b, int32_t trial_ticks, int32_t *seg, int32_t *threads, int32_t *groups,
                         int32_t *mode, double *best_ms);
 
+/* ---- rendering ---------------------------------------------------------------- */
+/* Pixel -> ray as the reference's full-screen ray marcher
+ * (shaders/fs_raymarch.frag:67-75, shaders/fs_raymarch.vert:30-36, quad UVs
+ * src/engine/rendering/renderer.cpp:1253-1267): py = 0 is the top row; camera
+ * +x is screen-left, +y screen-down, +z forward; uv.y is divided by aspect. */
+typedef struct fs3d_camera {
+  float origin[3];
+  float basis[9]; /* screen-left, screen-down, forward: world dir = dx*b0 + dy*b1 + dz*b2 */
+  float light[3];
+  float aspect;
+} fs3d_camera;
+/* HIP port of shaders/fs_raymarch.frag (sphere tracer; origin = cam_pos as in
+ * shaders/fs_raymarch.vert:33): rgba[H][W][4] float, written to host memory. */
+int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, float *rgba);
+/* Voxel ray caster over the world's current grid (the grid -> renderer hand-off
+ * the reference's set-2 bindings would carry, src/engine/rendering/materials.cpp:26-59):
+ * rgb[H][W][3] float, hit[H][W] = linear voxel index of the first non-EMPTY cell
+ * (UINT64_MAX on a miss), face[H][W] = entry face (0 -x, 1 +x, 2 -y, 3 +y, 4 -z,
+ * 5 +z, 6 started inside, 255 miss).  Colours from the reference palette
+ * (src/engine/rendering/renderer.cpp:136-393): STATIC 127, SAND 162, FLUID 137,
+ * other bytes their own index.  Synchronises. */
+int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t H, float *rgb, uint64_t *hit,
+                      uint8_t *face);
+int fs3d_world_last_render_ms(const fs3d_world *w, double *ms);
+/* Seeded-orbit camera: eye at angle/elevation (radians) around the grid centre,
+ * dist_scale * max(dims) away, looking at the centre with +z up on screen. */
+int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, double elev, float aspect,
+                      fs3d_camera *out);
+/* The reference's GlobalData block (src/engine/rendering/materials.h:19-28) as
+ * Material::writeGlobalAndObjectData fills it (src/engine/rendering/materials.cpp:531-545):
+ * 76 floats = camPos, aspect, camDir (vec4 each), view, proj, viewproj,
+ * invViewProj (column-major mat4 each). */
+int fs3d_camera_globaldata(const float cam_pos[3], const float cam_rot[3], float *block);
+/* One fixed 0.016 s step of the reference's key-driven camera
+ * (src/engine/rendering/renderer.cpp:437-467). */
+enum {
+  FS3D_KEY_W = 1, FS3D_KEY_A = 2, FS3D_KEY_S = 4, FS3D_KEY_D = 8,
+  FS3D_KEY_CTRL = 16, FS3D_KEY_SPACE = 32, FS3D_KEY_LEFT = 64, FS3D_KEY_RIGHT = 128
+};
+int fs3d_camera_integrate(float cam_pos[3], float cam_rot[3], uint32_t keys);
+/* The 256-entry RGBA palette (rgba[256*4]). */
+int fs3d_palette(float *rgba);
+
 /* ---- serving ----------------------------------------------------------------- */
 /* Continuous batching of simulation requests.  No reference counterpart (the
  * reference is one interactive window, src/engine/engine.cpp:54-74).  Requests

@@ -106,6 +106,20 @@ void fso_serve_specs(int n, uint64_t seed, int lo, int hi, int tick_lo, int tick
                      int32_t *dims, uint64_t *thr, uint64_t *seeds, int32_t *ticks,
                      double *arrival);
 
+/* ---- render (fs3d_oracle_render.c) ----------------------------------------- */
+typedef struct {
+  float origin[3];
+  float basis[9]; /* screen-left, screen-down, forward (world dir = dx*b0 + dy*b1 + dz*b2) */
+  float light[3];
+  float aspect;
+} fso_camera;
+void fso_pixel_dir(int px, int py, int W, int H, float aspect, float d[3]);
+void fso_raymarch_sdf(const float cam_pos[3], float aspect, int W, int H, float *rgba);
+void fso_render_dda(const uint8_t *g, int nx, int ny, int nz, const fso_camera *cam, const float *palette, int W,
+                    int H, float *rgb, uint64_t *hit, uint8_t *face);
+void fso_orbit_camera(int nx, int ny, int nz, double angle, double dist_scale, double elev, float aspect,
+                      fso_camera *out);
+
 #ifdef __cplusplus
 }
 #endif

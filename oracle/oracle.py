@@ -30,6 +30,14 @@ class PillarParams(ctypes.Structure):
                  "pool_d_min", "pool_d_max", "pool_h_min", "pool_h_max")]
 
 
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("basis", ctypes.c_float * 9), ("light", ctypes.c_float * 3),
+                ("aspect", ctypes.c_float)]
+
+
+f32p = ctypes.POINTER(ctypes.c_float)
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -73,6 +81,12 @@ def lib():
             "fso_serve_specs": (None, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, i32p, u64p, u64p,
                                        i32p, f64p]),
+            "fso_pixel_dir": (None, [ctypes.c_int] * 4 + [ctypes.c_float, f32p]),
+            "fso_raymarch_sdf": (None, [f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int, f32p]),
+            "fso_render_dda": (None, [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Camera), f32p,
+                                      ctypes.c_int, ctypes.c_int, f32p, u64p, u8p]),
+            "fso_orbit_camera": (None, [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_float,
+                                                                                    ctypes.POINTER(Camera)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -226,3 +240,45 @@ def serve_specs(n, seed, lo=32, hi=256, tick_lo=10, tick_hi=500, lam=2000.0):
     lib().fso_serve_specs(n, seed, lo, hi, tick_lo, tick_hi, lam, _p(dims, i32p), _p(thr, u64p),
                           _p(seeds, u64p), _p(ticks, i32p), _p(arr, f64p))
     return dims.reshape(n, 3), thr, seeds, ticks, arr
+
+
+def pixel_dir(px, py, W, H, aspect):
+    d = np.zeros(3, np.float32)
+    lib().fso_pixel_dir(px, py, W, H, aspect, _p(d, f32p))
+    return d
+
+
+def raymarch_sdf(cam_pos, aspect, W, H):
+    """RGBA float32 image (H, W, 4) of the fs_raymarch sphere tracer."""
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    lib().fso_raymarch_sdf(_p(cp, f32p), aspect, W, H, _p(out, f32p))
+    return out
+
+
+def camera_from(origin, basis, light, aspect):
+    c = Camera()
+    c.origin[:] = [float(v) for v in origin]
+    c.basis[:] = [float(v) for v in np.asarray(basis).reshape(9)]
+    c.light[:] = [float(v) for v in light]
+    c.aspect = float(aspect)
+    return c
+
+
+def orbit_camera(dims, angle, dist_scale, elev, aspect):
+    c = Camera()
+    lib().fso_orbit_camera(int(dims[0]), int(dims[1]), int(dims[2]), angle, dist_scale, elev, aspect, ctypes.byref(c))
+    return c
+
+
+def render_dda(grid, cam, palette, W, H):
+    """(rgb (H,W,3) float32, hit (H,W) uint64, face (H,W) uint8) of the voxel DDA renderer."""
+    g = np.ascontiguousarray(grid, np.uint8)
+    nz, ny, nx = g.shape
+    pal = np.ascontiguousarray(palette, np.float32).reshape(256 * 4)
+    rgb = np.zeros((H, W, 3), np.float32)
+    hit = np.zeros((H, W), np.uint64)
+    face = np.zeros((H, W), np.uint8)
+    lib().fso_render_dda(_p(g, u8p), nx, ny, nz, ctypes.byref(cam), _p(pal, f32p), W, H, _p(rgb, f32p),
+                         _p(hit, u64p), _p(face, u8p))
+    return rgb, hit, face
