@@ -71,6 +71,47 @@ def bench_pillars2048(F, ticks):
         return dict(vox=n ** 3, wall=wall, kern_ms=kern_ms, ticks=ticks, hash=int(h), seg=w.segment)
 
 
+def orbit_angles(seed, n):
+    """Seeded orbit-camera angles: 2*pi * (SplitMix64 output >> 11) / 2^53."""
+    import math
+    st, out = seed, []
+    for _ in range(n):
+        st = (st + 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
+        z ^= z >> 31
+        out.append(2.0 * math.pi * (z >> 11) / float(1 << 53))
+    return out
+
+
+def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
+    """The 2048^3 collapsing-column scene: `ticks` ticks in graphs of `every`,
+    a W x H voxel render after each, from a seeded orbit camera."""
+    from fallingsand3d_amd import render as R
+    n = 2048
+    frames, tick_ms, render_ms = [], [], []
+    with F.World((n, n, n)) as w:
+        w.fill_pillars(seed)
+        w.run(every, 0)  # capture the graph
+        w.fill_pillars(seed)
+        w.tick_count = 0
+        angles = orbit_angles(seed, ticks // every)
+        t0 = time.perf_counter()
+        for k, ang in enumerate(angles):
+            w.run(every, 0)
+            tick_ms.append(w.last_run_ms_per_tick())
+            cam = R.orbit_camera(w.dims, ang, 1.4, 0.35, W / H)
+            rgb, hit, face = R.render(w, cam, W, H)
+            render_ms.append(R.last_render_ms(w))
+            frames.append(float((hit != R.MISS).mean()))
+        wall = time.perf_counter() - t0
+        h = w.hash()
+    return dict(wall=wall, tick_ms=sum(tick_ms) / len(tick_ms), render_ms=sum(render_ms) / len(render_ms),
+                render_ms_max=max(render_ms), hit_frac=sum(frames) / len(frames), frames=len(frames), hash=int(h),
+                vox=n ** 3, ticks=ticks, rays=W * H)
+
+
 def bench_copy(F, nbytes, iters=20):
     ms = F.copy_probe(nbytes, iters)
     log(f"[bench] copy: {nbytes} bytes in {ms:.4f} ms ({2 * nbytes / ms / 1e6:.1f} GB/s)")
@@ -169,7 +210,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving"], default=None)
+    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving", "render2048"],
+                    default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -193,6 +235,10 @@ def main():
     if args.only == "batched":
         bb = bench_batched(F, ticks=args.steps)
         print(json.dumps({"workload": "batched", **bb}), flush=True)
+        return
+    if args.only == "render2048":
+        pr = bench_pillars_render(F)
+        print(json.dumps({"workload": "render2048", **pr}), flush=True)
         return
     if args.only == "serving":
         sv = bench_serving(F)
@@ -230,6 +276,14 @@ def main():
         p = bench_pillars2048(F, 200)
         ach2, frac2 = roofline(p["vox"], p["kern_ms"])
         c = bench_copy(F, 8 << 30)
+        pr = bench_pillars_render(F)
+        extra["pillars2048_render"] = {
+            "workload": "2048^3 pillars, 200 ticks in graphs of 10, 1920x1080 voxel render after each (20 frames), "
+                        "seeded orbit camera",
+            "wall_s": round(pr["wall"], 3), "tick_ms": round(pr["tick_ms"], 4),
+            "render_ms_mean": round(pr["render_ms"], 3), "render_ms_max": round(pr["render_ms_max"], 3),
+            "rays_per_s": round(pr["rays"] / (pr["render_ms"] * 1e-3), 1), "hit_fraction": round(pr["hit_frac"], 4),
+            "final_hash": pr["hash"]}
         extra["pillars2048"] = {
             "workload": "2048^3 uint8, 256 seeded sand pillars in a fluid pool, 200 plain ticks in one graph",
             "gvox_per_s": round(p["vox"] * p["ticks"] / p["wall"] / 1e9, 2),

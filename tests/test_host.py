@@ -88,3 +88,23 @@ def test_last_error_is_thread_local_message():
     with pytest.raises(F.Fs3dError) as ei:
         F.World((-1, 2, 2))
     assert "dims" in str(ei.value)
+
+
+def _build_wrapper_demo(tmp_path):
+    import subprocess
+    exe = tmp_path / "wrapper_demo"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "wrapper_demo.cpp"), "-o", str(exe),
+                    "-L", os.path.dirname(_lib.LIB_PATH), "-lfs3d",
+                    "-Wl,-rpath," + os.path.dirname(_lib.LIB_PATH)], check=True)
+    return exe
+
+
+def test_cpp_wrapper_rethrows_without_gpu(tmp_path):
+    import subprocess
+    if F.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    exe = _build_wrapper_demo(tmp_path)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "no-device error" in r.stdout and "ERROR: " in r.stderr

@@ -182,3 +182,16 @@ def test_boxes512_long_run_properties(gpu):
             hh.append(v.tick())
         assert [int(x) for x in h] == hh
         assert (w.counts() == c0).all()
+
+
+def test_cpp_wrapper_on_gpu(gpu, tmp_path):
+    import subprocess
+    from test_host import _build_wrapper_demo
+    path = os.path.join(os.path.dirname(__file__), "golden", "rain64_hashes.txt")
+    want = [int(x) for x in open(path).read().split("\n")[1:] if x.strip()]
+    announce("C++ wrapper: 64^3 rain, 256 ticks")
+    exe = _build_wrapper_demo(tmp_path)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.split("\n") if ln.startswith("hash ")][0]
+    assert int(line.split()[1]) == want[256]
