@@ -111,6 +111,11 @@ void release(fs3d_world *w) {
   if (w->d_hit) (void)hipFree(w->d_hit);
   if (w->d_face) (void)hipFree(w->d_face);
   if (w->d_small) (void)hipFree(w->d_small);
+  for (int i = 0; i < 2; i++)
+    if (w->d_masks[i]) (void)hipFree(w->d_masks[i]);
+  if (w->d_stats) (void)hipFree(w->d_stats);
+  if (w->d_list) (void)hipFree(w->d_list);
+  if (w->d_ctl) (void)hipFree(w->d_ctl);
   if (w->h_pinned) (void)hipHostFree(w->h_pinned);
   if (w->ev0) (void)hipEventDestroy(w->ev0);
   if (w->ev1) (void)hipEventDestroy(w->ev1);
@@ -118,14 +123,72 @@ void release(fs3d_world *w) {
   delete w;
 }
 
+constexpr size_t kStatWords = (size_t)kHashParts * kHashStride + 1;
+
+// Every change bit set: the next two ticks evaluate every segment, after which
+// the masks describe real changes again.
+int mark_all(fs3d_world *w) {
+  for (int i = 0; i < 2; i++) FS3D_HIP(launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream));
+  // work-list counts start from zero after any host-side change (tick, shape)
+  FS3D_HIP(hipMemsetAsync(w->d_ctl, 0, 4 * sizeof(u32), w->stream));
+  return FS3D_OK;
+}
+
+// Sets the change bits of every lane segment whose blocks contain cell (x,y,z),
+// at both parities (segments x/S - 1 .. x/S + 1 of the row quads holding it).
+int mark_cell(fs3d_world *w, int x, int y, int z) {
+  long long idx[2];
+  for (int q = 0; q < 2; q++) {
+    const long long nyq = (w->ny + q + 1) >> 1;
+    idx[q] = (long long)((z + q) >> 1) * nyq + ((y + q) >> 1);
+  }
+  const int s = x / w->shape.seg, lr = w->shape.lanes_per_row;
+  u64 bits = 1ull << s;  // s <= 63
+  if (s > 0) bits |= 1ull << (s - 1);
+  if (s + 1 < lr) bits |= 1ull << (s + 1);
+  FS3D_HIP(launch_mark_cells(w->d_masks[0], w->d_masks[1], idx[0], idx[1], bits, w->stream));
+  return FS3D_OK;
+}
+
+// Persistent waves of the listed tick: as many as stay resident at its register
+// use, no more than one pass over every row quad needs.
+int size_list(fs3d_world *w) {
+  int cus = 0;
+  FS3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w->dev));
+  const long long need = ((long long)w->mask_words + w->shape.rq_per_wave - 1) / w->shape.rq_per_wave;
+  const long long resident = (long long)std::max(cus, 1) * list_blocks_per_cu(w->shape.seg) * 4;
+  long long lw = std::min<long long>(need, resident);
+  lw = std::max<long long>((lw + 3) & ~3LL, 4);
+  w->list_waves = (u32)lw;
+  return FS3D_OK;
+}
+
+void drop_graphs(fs3d_world *w) {
+  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+  w->graphs.clear();
+}
+
+bool skip_listed(const fs3d_world *w) {
+  return w->skip == FS3D_SKIP_LISTED || (w->skip == FS3D_SKIP_AUTO && w->shape.waves > (1ll << 17));
+}
+
 int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev) {
   const u64 *rule = device_rule_table(w->dev);
+  u64 *const *masks = w->skip ? w->d_masks : nullptr;
+  const bool listed = skip_listed(w);
   for (int k = 0; k < nticks; k++) {
     const bool hashed = hash_every > 0 && (k + 1) % hash_every == 0;
     const int src = (w->cur + k) & 1;
+    if (masks && listed && !hashed) {
+      // settled row quads are listed out; hash ticks read every segment anyway
+      FS3D_HIP(launch_tick_listed(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                                  w->tick + (u64)k, tick_dev, (u32)k, rule, w->stream, masks,
+                                  w->d_stats, w->d_list, w->d_ctl, w->mask_words, w->list_waves));
+      continue;
+    }
     FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
-                         (u32)(w->tick + (u64)k), tick_dev, (u32)k, rule,
-                         hashed ? w->d_parts : nullptr, w->stream));
+                         w->tick + (u64)k, tick_dev, (u32)k, rule,
+                         hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
     if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
   }
   return FS3D_OK;
@@ -160,8 +223,8 @@ int fs3d_rule_lut(uint16_t *lut) {
 }
 
 int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
-  (void)flags;
   if (!dims || !out) return set_error(FS3D_ERR_INVALID, "dims/out is NULL");
+  if (flags & ~(uint32_t)FS3D_WORLD_DENSE) return set_error(FS3D_ERR_INVALID, "unknown flags 0x%x", flags);
   *out = nullptr;
   for (int a = 0; a < 3; a++)
     if (dims[a] < 1) return set_error(FS3D_ERR_INVALID, "dims must be >= 1 (got %d,%d,%d)", dims[0], dims[1], dims[2]);
@@ -179,6 +242,9 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   w->nz = dims[2];
   w->n = n;
   w->alloc = (n + kGuardBytes + 255) & ~(size_t)255;
+  w->mask_words = (size_t)((w->ny + 2) >> 1) * (size_t)((w->nz + 2) >> 1);
+  // work-list counts are 32-bit: beyond 2^31 row quads a world ticks densely
+  w->skip = (flags & FS3D_WORLD_DENSE) || w->mask_words >= (1ull << 31) ? FS3D_SKIP_OFF : FS3D_SKIP_AUTO;
   rc = plan_tick(w->nx, w->ny, w->nz, 0, &w->shape);
   if (rc) {
     delete w;
@@ -186,6 +252,9 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->buf[i], w->alloc);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->d_masks[i], w->mask_words * sizeof(u64));
+  if (e == hipSuccess) e = hipMalloc(&w->d_list, w->mask_words * sizeof(TickEntry));
+  if (e == hipSuccess) e = hipMalloc(&w->d_ctl, 4 * sizeof(u32));
   if (e == hipSuccess) e = hipMalloc(&w->d_parts, (size_t)kHashParts * kHashStride * sizeof(u64));
   if (e == hipSuccess) e = hipEventCreate(&w->ev0);
   if (e == hipSuccess) e = hipEventCreate(&w->ev1);
@@ -198,12 +267,16 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   }
   e = hipMemsetAsync(w->d_parts, 0, (size_t)kHashParts * kHashStride * sizeof(u64), w->stream);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMemsetAsync(w->buf[i], 0, w->alloc, w->stream);
+  for (int i = 0; i < 2 && e == hipSuccess; i++)
+    e = launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(w->d_ctl, 0, 4 * sizeof(u32), w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   if (e != hipSuccess) {
     release(w);
     return set_error(FS3D_ERR_HIP, "world init failed: %s", hipGetErrorString(e));
   }
   rc = ensure_small(w, 64);
+  if (!rc) rc = size_list(w);
   if (rc) {
     release(w);
     return rc;
@@ -239,6 +312,7 @@ int fs3d_world_set_voxel(fs3d_world *w, int32_t x, int32_t y, int32_t z, uint8_t
   if (int rc = voxel_offset(w, x, y, z, &off)) return rc;
   FS3D_HIP(hipSetDevice(w->dev));
   FS3D_HIP(hipMemcpyAsync(w->buf[w->cur] + off, &v, 1, hipMemcpyHostToDevice, w->stream));
+  if (int rc = mark_cell(w, x, y, z)) return rc;
   FS3D_HIP(hipStreamSynchronize(w->stream));
   return clear_error();
 }
@@ -259,6 +333,7 @@ int fs3d_world_upload(fs3d_world *w, const uint8_t *host, size_t nbytes) {
   if (!host || nbytes != w->n) return set_error(FS3D_ERR_INVALID, "upload needs %zu bytes (got %zu)", w->n, nbytes);
   FS3D_HIP(hipSetDevice(w->dev));
   FS3D_HIP(hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream));
+  if (int rc = mark_all(w)) return rc;
   FS3D_HIP(hipStreamSynchronize(w->stream));
   return clear_error();
 }
@@ -281,6 +356,8 @@ int fs3d_world_get_tick(const fs3d_world *w, uint64_t *tick) {
 
 int fs3d_world_set_tick(fs3d_world *w, uint64_t tick) {
   if (int rc = check_world(w)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (int rc = mark_all(w)) return rc;
   w->tick = tick;
   return clear_error();
 }
@@ -293,8 +370,61 @@ int fs3d_world_set_segment(fs3d_world *w, int32_t seg) {
   FS3D_HIP(hipStreamSynchronize(w->stream));
   w->shape = sh;
   w->seg_req = seg;
-  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
-  w->graphs.clear();
+  drop_graphs(w);
+  if (int rc = size_list(w)) return rc;
+  if (int rc = mark_all(w)) return rc;  // the mask bits index lane segments
+  return clear_error();
+}
+
+int fs3d_world_set_skip(fs3d_world *w, int32_t mode) {
+  if (int rc = check_world(w)) return rc;
+  if (mode < FS3D_SKIP_OFF || mode > FS3D_SKIP_WAVES) return set_error(FS3D_ERR_INVALID, "skip mode %d", mode);
+  if (mode && w->mask_words >= (1ull << 31))
+    return set_error(FS3D_ERR_INVALID, "skipping needs fewer than 2^31 row quads (ny*nz/4)");
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (mode != w->skip) {
+    FS3D_HIP(hipStreamSynchronize(w->stream));
+    drop_graphs(w);
+    w->skip = mode;
+  }
+  if (int rc = mark_all(w)) return rc;
+  return clear_error();
+}
+
+int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode) {
+  if (int rc = check_world(w)) return rc;
+  if (!mode) return set_error(FS3D_ERR_INVALID, "mode is NULL");
+  *mode = w->skip;
+  return clear_error();
+}
+
+int fs3d_world_sweep_stats(fs3d_world *w, int32_t enable, uint64_t out[3]) {
+  if (int rc = check_world(w)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  u64 tot[3] = {0, 0, 0};
+  if (w->d_stats) {
+    std::vector<u64> h(kStatWords);
+    FS3D_HIP(hipMemcpy(h.data(), w->d_stats, kStatWords * sizeof(u64), hipMemcpyDeviceToHost));
+    for (int i = 0; i < kHashParts; i++) {
+      tot[0] += h[(size_t)i * kHashStride];
+      tot[1] += h[(size_t)i * kHashStride + 1];
+    }
+    tot[2] = h[kStatWords - 1];
+  }
+  if (enable && !w->d_stats) {
+    FS3D_HIP(hipMalloc(&w->d_stats, kStatWords * sizeof(u64)));
+    drop_graphs(w);
+  } else if (!enable && w->d_stats) {
+    FS3D_HIP(hipFree(w->d_stats));
+    w->d_stats = nullptr;
+    drop_graphs(w);
+  }
+  if (w->d_stats) {
+    FS3D_HIP(hipMemset(w->d_stats, 0, kStatWords * sizeof(u64)));
+  }
+  if (out)
+    for (int i = 0; i < 3; i++) out[i] = tot[i];
   return clear_error();
 }
 
@@ -421,6 +551,7 @@ int fs3d_world_fill_rain(fs3d_world *w, uint64_t seed, uint64_t thr) {
   if (int rc = check_world(w)) return rc;
   FS3D_HIP(hipSetDevice(w->dev));
   FS3D_HIP(launch_fill_rain(w->buf[w->cur], w->nx, w->ny, w->nz, seed, thr, w->stream));
+  if (int rc = mark_all(w)) return rc;
   FS3D_HIP(hipStreamSynchronize(w->stream));
   return clear_error();
 }
@@ -437,6 +568,7 @@ int fs3d_world_fill_boxes(fs3d_world *w, uint64_t seed, const fs3d_box_params *p
     return set_error(FS3D_ERR_INVALID, "box parameters do not fit the grid");
   }
   hipError_t e = hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   (void)hipHostFree(host);
   if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "box upload failed: %s", hipGetErrorString(e));
@@ -454,6 +586,7 @@ int fs3d_world_fill_pillars(fs3d_world *w, uint64_t seed, const fs3d_pillar_para
   FS3D_HIP(hipSetDevice(w->dev));
   u8 *g = w->buf[w->cur];
   const size_t layer = (size_t)w->nx * w->ny;
+  if (int rc = mark_all(w)) return rc;
   FS3D_HIP(hipMemsetAsync(g, FS3D_EMPTY, w->n, w->stream));
   FS3D_HIP(hipMemsetAsync(g, FS3D_STATIC, layer, w->stream));
   if (pool[4] >= 1)

@@ -46,7 +46,9 @@ u64 prob_threshold(u64 num, u64 den);
 int loguniform_int(u64 r, int lo, int hi);
 
 // ---- rule table ------------------------------------------------------------------
-// entries[v*65536 + s]: byte b of the u64 is the output category of cell b.
+// entries[v*65536 + s]: byte b of the u64 is the output category of cell b;
+// bit 7 of byte 0 (kRuleChanged) is set when the block changes.
+constexpr u64 kRuleChanged = 0x80ull;
 void build_rule_entries(u64 *entries);
 u16 rule_state_packed(u16 s, int v);
 
@@ -63,9 +65,37 @@ struct TickShape {
   unsigned blocks;    // workgroups of 4 waves
 };
 int plan_tick(int nx, int ny, int nz, int seg_req, TickShape *out);
+// masks (optional): the two per-parity change-mask arrays, one u64 per row quad
+// of the larger parity; when given, settled lane segments are skipped.  stats
+// (optional, with masks): kHashParts slots of {swept voxels, read-only voxels}
+// plus one violation counter.
+// ctl (with masks): the listed tick's work-list counts, which a hash tick keeps
+// consistent (see launch_tick_listed).
 hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz,
-                       u32 t, const u64 *tick_dev, u32 k_off, const u64 *rule,
-                       u64 *hash_parts, hipStream_t s);
+                       u64 t, const u64 *tick_dev, u32 k_off, const u64 *rule,
+                       u64 *hash_parts, hipStream_t s, u64 *const *masks = nullptr, u64 *stats = nullptr,
+                       u32 *ctl = nullptr);
+hipError_t launch_fill_u64(u64 *p, size_t n, u64 v, hipStream_t s);
+// Work-list entry of the listed skipping tick: active lanes of one row quad and
+// its position (zq << 32 | yq).
+struct TickEntry {
+  u64 act;
+  u64 pos;
+};
+// Skipping tick in two launches: a plan pass over all row quads that lists
+// those with active lanes (list capacity mask_words), then a persistent pass of
+// list_waves waves (a multiple of 4) over the list.  No hash.  ctl holds three
+// list counts, used by ticks t mod 3; the plan pass of tick t (or the hash tick
+// at t) clears slot (t+1) mod 3, so all three must be zero before the first
+// tick after any change of the tick counter.
+hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
+                              const u64 *tick_dev, u32 k_off, const u64 *rule, hipStream_t s,
+                              u64 *const *masks, u64 *stats, TickEntry *list, u32 *ctl, size_t mask_words,
+                              u32 list_waves);
+// Resident 256-thread workgroups per compute unit of the listed tick's
+// persistent kernel for a segment size.
+int list_blocks_per_cu(int seg);
+hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s);
 hipError_t launch_hash(const u8 *g, size_t n, u64 *hash_parts, hipStream_t s);
 hipError_t launch_hash_finalize(u64 *hash_parts, u64 *out, hipStream_t s);
 hipError_t launch_set_u64(u64 *p, u64 v, hipStream_t s);
@@ -112,6 +142,15 @@ struct fs3d_world {
   fs3d::u8 *d_face = nullptr;
   size_t render_px = 0;
   double last_render_ms = 0.0;
+  // settled-segment skipping: mode (FS3D_SKIP_*) and per-parity change masks
+  // (one u64 per row quad)
+  int skip = FS3D_SKIP_AUTO;
+  fs3d::u64 *d_masks[2] = {nullptr, nullptr};
+  size_t mask_words = 0;
+  fs3d::u64 *d_stats = nullptr;  // sweep counters, allocated when enabled
+  fs3d::TickEntry *d_list = nullptr;  // work list of the listed tick (mask_words entries)
+  fs3d::u32 *d_ctl = nullptr;         // list counts, one per tick mod 3
+  fs3d::u32 list_waves = 0;           // persistent waves of the listed tick
   // cached graphs keyed by (nticks, hash_every, cur)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 };

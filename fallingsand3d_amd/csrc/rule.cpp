@@ -111,7 +111,12 @@ void build_rule_entries(u64 *entries) {
     for (u32 s = 0; s < 65536u; s++) {
       Masks m = evaluate((u16)s, v);
       u64 e = 0;
-      for (int b = 0; b < 8; b++) e |= (u64)m.get(b) << (8 * b);
+      u32 r = 0;
+      for (int b = 0; b < 8; b++) {
+        e |= (u64)m.get(b) << (8 * b);
+        r |= (u32)m.get(b) << (2 * b);
+      }
+      if (r != s) e |= kRuleChanged;
       entries[(size_t)v * 65536u + s] = e;
     }
 }

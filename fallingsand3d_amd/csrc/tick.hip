@@ -21,25 +21,69 @@ struct TickArgs {
   u8 *dst;
   int nx, ny, nz;
   int lanes_per_row, rq_per_wave;
-  u32 t;
+  u64 t;
   u32 k_off;
   const u64 *tick_dev;  // when set, tick = *tick_dev + k_off (graph replays)
   const u64 *rule;      // 4 x 65536 entries
   u64 *parts;           // hash partials (HASH only)
+  u64 *masks[2];        // per-parity change masks (SKIP only)
+  u64 *stats;           // optional sweep counters (SKIP only): kHashParts slots + violations
+  u32 *ctl;             // work-list counts, one per tick mod 3 (SKIP only)
 };
 
-template <int S, bool HASH>
+__device__ __forceinline__ u64 tick_of(const TickArgs &a) { return a.tick_dev ? *a.tick_dev + a.k_off : a.t; }
+
+template <int S, bool HASH, bool SKIP>
 __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   const int lane = threadIdx.x & 63;
   const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const u32 t = a.tick_dev ? (u32)(*a.tick_dev + a.k_off) : a.t;
+  const u64 t64 = tick_of(a);
+  const u32 t = (u32)t64;
   const int p = (int)(t & 1u);
   const long long nrq = (long long)((a.ny + p + 1) >> 1) * ((a.nz + p + 1) >> 1);
+  // A hash tick replaces the plan pass of the listed tick: it clears the count
+  // slot the next tick's plan pass appends to.
+  if (SKIP && HASH && a.ctl && blockIdx.x == 0 && threadIdx.x == 0) a.ctl[(t64 + 1) % 3] = 0;
   // Whole waves past the end leave early; the hashing variant keeps them until
   // its workgroup barrier (they load and store nothing).
   if (!HASH && wave * a.rq_per_wave >= nrq) return;
-  u64 acc = tick_wave<S, HASH>(a.src, a.dst, a.nx, a.ny, a.nz, a.lanes_per_row, a.rq_per_wave, t, wave, lane,
-                               a.rule);
+  RowQuad<S> R;
+  SkipCtx sk;
+  if constexpr (SKIP) {
+    sk.mine = a.masks[p];
+    sk.other = a.masks[p ^ 1];
+    sk.nyq_o = (a.ny + (p ^ 1) + 1) >> 1;
+    sk.nzq_o = (a.nz + (p ^ 1) + 1) >> 1;
+    sk.load_skipped = HASH;  // hash ticks read settled segments to hash them
+  }
+  rq_load<S, SKIP>(R, a.src, a.nx, a.ny, a.nz, a.lanes_per_row, a.rq_per_wave, p, wave, lane, &sk);
+  const u64 lane_bits = a.lanes_per_row == 64 ? ~0ull : ((1ull << a.lanes_per_row) - 1ull);
+  if constexpr (SKIP && !HASH) {
+    if (__ballot(R.active && !R.skip) == 0) {  // the whole wave is settled
+      if (R.active && R.seg == 0) sk.mine[R.q] = 0;
+      return;
+    }
+  }
+  bool ch = false;
+  u64 acc = rq_update<S, HASH, SKIP>(R, a.dst, a.lanes_per_row, t, a.rule, &ch);
+  if constexpr (SKIP) {
+    const u64 b = __ballot(ch);
+    if (R.active && R.seg == 0) sk.mine[R.q] = (b >> (lane - R.seg)) & lane_bits;
+    if (a.stats) {
+      const u64 vox = R.active ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
+      const u64 swept = wave_sum_u64(R.skip ? 0 : vox);
+      const u64 ronly = wave_sum_u64(R.skip && HASH ? vox : 0);
+      const u64 bad = __ballot(R.skip && ch);  // a settled segment changed: must never happen
+      if (lane == 0) {
+        u64 *slot = a.stats + (size_t)(wave % kHashParts) * kHashStride;
+        atomicAdd((unsigned long long *)slot, (unsigned long long)swept);
+        if (ronly) atomicAdd((unsigned long long *)(slot + 1), (unsigned long long)ronly);
+        if (bad)
+          atomicAdd((unsigned long long *)(a.stats + (size_t)kHashParts * kHashStride),
+                    (unsigned long long)__builtin_popcountll(bad));
+      }
+    }
+  }
   if constexpr (HASH) {
     __shared__ u64 wsum[4];
     acc = wave_sum_u64(acc);
@@ -49,6 +93,136 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
       const u64 tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
       if (tot) atomicAdd((unsigned long long *)(a.parts + (size_t)(blockIdx.x % kHashParts) * kHashStride),
                          (unsigned long long)tot);
+    }
+  }
+}
+
+// ---- listed (work-list) skipping tick ------------------------------------------
+// A plan pass turns the change masks into the lane-activity word of every row
+// quad, (mine | o | o << 1 | o >> 1) over the 64 lane bits, o being the OR of
+// the four other-parity masks that share its rows; row quads with no active lane
+// are settled (their mask is already 0 and stays so) and cost nothing more, the
+// others are appended to a work list.  A persistent kernel then streams the list.
+
+constexpr int kPlanPerThread = 4;  // row quads per plan thread: 1024 per workgroup
+
+// Appends the active row quads of a 1024-row-quad range to the list, in row-quad
+// order within the range, with one atomic per workgroup on the count slot of
+// this tick (tick mod 3); clears the slot of the next tick.
+__global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, u32 *ctl) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const u64 t64 = tick_of(a);
+  const int p = (int)(t64 & 1u);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(t64 + 1) % 3] = 0;
+  const int nyq = (a.ny + p + 1) >> 1, nzq = (a.nz + p + 1) >> 1;
+  const int nyq_o = (a.ny + (p ^ 1) + 1) >> 1, nzq_o = (a.nz + (p ^ 1) + 1) >> 1;
+  const long long nrq = (long long)nyq * nzq;
+  const u64 lane_bits = a.lanes_per_row == 64 ? ~0ull : ((1ull << a.lanes_per_row) - 1ull);
+  const u64 *other = a.masks[p ^ 1];
+  u64 act[kPlanPerThread], bal[kPlanPerThread];
+  int yq[kPlanPerThread], zq[kPlanPerThread];
+  __shared__ u32 wcount[kPlanPerThread * 4];
+  __shared__ u32 wbase;
+#pragma unroll
+  for (int i = 0; i < kPlanPerThread; i++) {
+    const long long q = (long long)blockIdx.x * (256 * kPlanPerThread) + i * 256 + threadIdx.x;
+    act[i] = 0;
+    yq[i] = zq[i] = 0;
+    if (q < nrq) {
+      yq[i] = (int)(q % nyq);
+      zq[i] = (int)(q / nyq);
+      u64 o = 0;
+#pragma unroll
+      for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+          const int yo = yq[i] + dy - p, zo = zq[i] + dz - p;
+          if (yo >= 0 && yo < nyq_o && zo >= 0 && zo < nzq_o) o |= other[(long long)zo * nyq_o + yo];
+        }
+      act[i] = (a.masks[p][q] | o | (o << 1) | (o >> 1)) & lane_bits;
+    }
+    bal[i] = __ballot(act[i] != 0);
+    if (lane == 0) wcount[i * 4 + wv] = (u32)__builtin_popcountll(bal[i]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 tot = 0;
+    for (int k = 0; k < kPlanPerThread * 4; k++) {  // exclusive prefix in (i, wave) order
+      const u32 c = wcount[k];
+      wcount[k] = tot;
+      tot += c;
+    }
+    wbase = tot ? atomicAdd(&ctl[t64 % 3], tot) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPlanPerThread; i++) {
+    if (act[i]) {
+      const u32 idx = wbase + wcount[i * 4 + wv] + (u32)__builtin_popcountll(bal[i] & ((1ull << lane) - 1ull));
+      list[idx].act = act[i];
+      list[idx].pos = ((u64)(u32)zq[i] << 32) | (u32)yq[i];
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void tick_list_kernel(TickArgs a, const TickEntry *__restrict__ list, u32 *ctl,
+                                                        u32 nwaves) {
+  const int lane = threadIdx.x & 63;
+  const u32 wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const u64 t64 = tick_of(a);
+  const u32 t = (u32)t64;
+  const int p = (int)(t & 1u);
+  const u32 count = ctl[t64 % 3];
+  const int LR = a.lanes_per_row, rqpw = a.rq_per_wave;
+  const int rql = lane / LR, seg = lane - rql * LR;
+  const int nyq = (a.ny + p + 1) >> 1;
+  const u64 lane_bits = LR == 64 ? ~0ull : ((1ull << LR) - 1ull);
+  u64 *mine = a.masks[p];
+  const u64 stride = (u64)nwaves * rqpw;
+  // Entries are prefetched one iteration ahead, so an item's chain is its data
+  // loads, the update and the stores, as in the one-row-quad-per-wave kernel.
+  u64 act_n = 0, pos_n = 0;
+  {
+    const u64 it = (u64)wave * rqpw + rql;
+    if (rql < rqpw && it < count) {
+      act_n = list[it].act;
+      pos_n = list[it].pos;
+    }
+  }
+  for (u64 base = (u64)wave * rqpw; base < count; base += stride) {
+    const bool active = rql < rqpw && base + rql < count;
+    const u64 act = act_n, pos = pos_n;
+    act_n = 0;
+    pos_n = 0;
+    const u64 itn = base + stride + rql;
+    if (rql < rqpw && itn < count) {
+      act_n = list[itn].act;
+      pos_n = list[itn].pos;
+    }
+    RowQuad<S> R;
+    // An opaque copy of the (loop-invariant) segment index: otherwise every
+    // per-lane predicate derived from it (byte-count masks of partial segments,
+    // edge tests) is hoisted out of the loop and held in SGPR pairs, which spill.
+    int seg_it = seg;
+    asm volatile("" : "+v"(seg_it));
+    R.seg = seg_it;
+    R.active = active;
+    R.yq = (int)(u32)pos;
+    R.zq = (int)(pos >> 32);
+    R.q = (long long)R.zq * nyq + R.yq;
+    R.skip = active && !((act >> seg) & 1u);
+    rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, !R.skip);
+    bool ch = false;
+    rq_update<S, false, true>(R, a.dst, LR, t, a.rule, &ch);
+    const u64 b = __ballot(ch);
+    if (active && seg == 0) mine[R.q] = (b >> (lane - seg)) & lane_bits;
+    if (a.stats) {
+      const u64 vox = active && !R.skip ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
+      const u64 swept = wave_sum_u64(vox);
+      if (lane == 0 && swept)
+        atomicAdd((unsigned long long *)(a.stats + (size_t)(wave % kHashParts) * kHashStride),
+                  (unsigned long long)swept);
     }
   }
 }
@@ -108,12 +282,28 @@ __global__ __launch_bounds__(256) void counts_kernel(const u8 *g, size_t n, u64 
 }
 
 template <int S>
-hipError_t launch_s(const TickShape &sh, const TickArgs &a, bool hash, hipStream_t s) {
-  if (hash)
-    hipLaunchKernelGGL((tick_kernel<S, true>), dim3(sh.blocks), dim3(256), 0, s, a);
+hipError_t launch_s(const TickShape &sh, const TickArgs &a, bool hash, bool skip, hipStream_t s) {
+  if (hash && skip)
+    hipLaunchKernelGGL((tick_kernel<S, true, true>), dim3(sh.blocks), dim3(256), 0, s, a);
+  else if (hash)
+    hipLaunchKernelGGL((tick_kernel<S, true, false>), dim3(sh.blocks), dim3(256), 0, s, a);
+  else if (skip)
+    hipLaunchKernelGGL((tick_kernel<S, false, true>), dim3(sh.blocks), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((tick_kernel<S, false>), dim3(sh.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tick_kernel<S, false, false>), dim3(sh.blocks), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void fill_u64_kernel(u64 *p, size_t n, u64 v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void mark_cells_kernel(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits) {
+  if (threadIdx.x == 0) {
+    m0[i0] |= bits;
+    m1[i1] |= bits;
+  }
 }
 
 }  // namespace
@@ -140,9 +330,9 @@ int plan_tick(int nx, int ny, int nz, int seg_req, TickShape *out) {
   return FS3D_OK;
 }
 
-hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u32 t,
+hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
                        const u64 *tick_dev, u32 k_off, const u64 *rule, u64 *hash_parts,
-                       hipStream_t s) {
+                       hipStream_t s, u64 *const *masks, u64 *stats, u32 *ctl) {
   TickArgs a;
   a.src = src;
   a.dst = dst;
@@ -156,13 +346,76 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
   a.tick_dev = tick_dev;
   a.rule = rule;
   a.parts = hash_parts;
+  a.masks[0] = masks ? masks[0] : nullptr;
+  a.masks[1] = masks ? masks[1] : nullptr;
+  a.stats = masks ? stats : nullptr;
+  a.ctl = masks ? ctl : nullptr;
   const bool hash = hash_parts != nullptr;
+  const bool skip = masks != nullptr;
   switch (sh.seg) {
-    case 8: return launch_s<8>(sh, a, hash, s);
-    case 16: return launch_s<16>(sh, a, hash, s);
-    case 32: return launch_s<32>(sh, a, hash, s);
-    default: return launch_s<64>(sh, a, hash, s);
+    case 8: return launch_s<8>(sh, a, hash, skip, s);
+    case 16: return launch_s<16>(sh, a, hash, skip, s);
+    case 32: return launch_s<32>(sh, a, hash, skip, s);
+    default: return launch_s<64>(sh, a, hash, skip, s);
   }
+}
+
+hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
+                              const u64 *tick_dev, u32 k_off, const u64 *rule, hipStream_t s,
+                              u64 *const *masks, u64 *stats, TickEntry *list, u32 *ctl, size_t mask_words,
+                              u32 list_waves) {
+  TickArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.lanes_per_row = sh.lanes_per_row;
+  a.rq_per_wave = sh.rq_per_wave;
+  a.t = t;
+  a.k_off = k_off;
+  a.tick_dev = tick_dev;
+  a.rule = rule;
+  a.parts = nullptr;
+  a.masks[0] = masks[0];
+  a.masks[1] = masks[1];
+  a.stats = stats;
+  a.ctl = ctl;
+  const size_t plan_blocks = (mask_words + 256 * kPlanPerThread - 1) / (256 * kPlanPerThread);
+  hipLaunchKernelGGL(plan_kernel, dim3((unsigned)plan_blocks), dim3(256), 0, s, a, list, ctl);
+  const dim3 grid(list_waves / 4);
+  switch (sh.seg) {
+    case 8: hipLaunchKernelGGL((tick_list_kernel<8>), grid, dim3(256), 0, s, a, list, ctl, list_waves); break;
+    case 16: hipLaunchKernelGGL((tick_list_kernel<16>), grid, dim3(256), 0, s, a, list, ctl, list_waves); break;
+    case 32: hipLaunchKernelGGL((tick_list_kernel<32>), grid, dim3(256), 0, s, a, list, ctl, list_waves); break;
+    default: hipLaunchKernelGGL((tick_list_kernel<64>), grid, dim3(256), 0, s, a, list, ctl, list_waves); break;
+  }
+  return hipGetLastError();
+}
+
+int list_blocks_per_cu(int seg) {
+  int n = 0;
+  hipError_t e;
+  switch (seg) {
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_list_kernel<8>, 256, 0); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_list_kernel<16>, 256, 0); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_list_kernel<32>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_list_kernel<64>, 256, 0); break;
+  }
+  return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+hipError_t launch_fill_u64(u64 *p, size_t n, u64 v, hipStream_t s) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s) {
+  hipLaunchKernelGGL(mark_cells_kernel, dim3(1), dim3(64), 0, s, m0, m1, i0, i1, bits);
+  return hipGetLastError();
 }
 
 hipError_t launch_hash(const u8 *g, size_t n, u64 *parts, hipStream_t s) {

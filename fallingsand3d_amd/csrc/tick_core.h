@@ -133,22 +133,71 @@ __device__ __forceinline__ u32 pack_states(u32 c0, u32 c1, u32 c2, u32 c3) {
   return __builtin_amdgcn_perm(b, a, 0x06020400u);
 }
 
+// ---- settled-segment skipping ----------------------------------------------------
+// A tick may skip a lane segment (no load, no store) when every block the lane
+// evaluates is known not to change.  Whether a block state is a fixed point of the
+// rule does not depend on the variant (a variant only picks among eligible
+// targets, never whether one exists), so a block that did not change at tick t-2
+// and none of whose cells changed at tick t-1 does not change at tick t; and the
+// destination buffer, which received tick t-2's output, already holds its bytes.
+// Changes are recorded per parity and row quad as a mask over lane segments: bit s
+// is set when lane s evaluated a block that changed.  At tick t (parity p) a lane
+// reads its own bit from the parity-p masks (written at tick t-2) and bits s-1..s+1
+// of the four parity-(1-p) row quads that share its rows (tick t-1), then rewrites
+// its row quad's parity-p mask.  Host-side edits set the bits of the cells they
+// touch (or every bit), so edited cells are recomputed at both parities.
+struct SkipCtx {
+  u64 *mine;          // parity-p masks: read (tick t-2), then rewritten (tick t)
+  const u64 *other;   // parity-(1-p) masks (tick t-1)
+  int nyq_o, nzq_o;   // row-quad counts of parity 1-p
+  bool load_skipped;  // skipped lanes still load (hash ticks); they never store
+};
+
 // One lane's share of one row quad: its S-byte segment of the 4 rows (r = dy +
 // 2*dz), loaded, plus where it lives.  Out-of-grid rows read as STATIC.
 template <int S>
 struct RowQuad {
   u32 own[4][S / 4];
   size_t rowoff[4];
-  int valid;  // bit r: row r is inside the grid
-  int m;      // valid bytes of the segment
+  long long q;  // row-quad index within the tick
+  int valid;    // bit r: row r is inside the grid
+  int m;        // valid bytes of the segment
   int seg, yq, zq;
+  bool active;  // the lane holds a row quad of this tick
+  bool skip;    // settled: nothing to evaluate or store
+  bool loaded;  // own[] holds the grid bytes (false: skipped without loading)
 };
 
-// Loads row-quad group `wave` (in units of rqpw row quads) of a tick of parity p.
+// Loads the lane's segment of the 4 rows of row quad (R.yq, R.zq) at parity p
+// (R.seg and R.active set); fetch = false leaves placeholders (STATIC).
 template <int S>
-__device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int LR, int rqpw,
-                                        int p, long long wave, int lane) {
+__device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int p, bool fetch) {
   constexpr int ND = S / 4;
+  R.loaded = fetch;
+  const int y0 = 2 * R.yq - p, z0 = 2 * R.zq - p;
+  const int xs = R.seg * S;
+  R.m = min(S, nx - xs);
+  R.valid = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int y = y0 + (r & 1), z = z0 + (r >> 1);
+    const bool ok = R.active && y >= 0 && y < ny && z >= 0 && z < nz;
+    R.valid |= ok ? (1 << r) : 0;
+    R.rowoff[r] = ((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs;
+#pragma unroll
+    for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
+    if (ok && fetch) load_seg<S>(src + R.rowoff[r], R.own[r]);
+    if (R.m < S) {
+#pragma unroll
+      for (int j = 0; j < ND; j++) R.own[r][j] = keep_bytes(R.own[r][j], R.m - 4 * j);
+    }
+  }
+}
+
+// Loads row-quad group `wave` (in units of rqpw row quads) of a tick of parity p.
+template <int S, bool SKIP = false>
+__device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int LR, int rqpw,
+                                        int p, long long wave, int lane, const SkipCtx *sk = nullptr) {
   const int nyq = (ny + p + 1) >> 1, nzq = (nz + p + 1) >> 1;
   const long long nrq = (long long)nyq * nzq;
   const int rql = lane / LR;
@@ -157,31 +206,34 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
   const bool active = rql < rqpw && q < nrq;
   R.yq = active ? (int)(q % nyq) : 0;
   R.zq = active ? (int)(q / nyq) : 0;
-  const int y0 = 2 * R.yq - p, z0 = 2 * R.zq - p;
-  const int xs = R.seg * S;
-  R.m = min(S, nx - xs);
-  R.valid = 0;
+  R.q = q;
+  R.active = active;
+  R.skip = false;
+  if constexpr (SKIP) {
+    if (active) {
+      u64 o = 0;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int y = y0 + (r & 1), z = z0 + (r >> 1);
-    const bool ok = active && y >= 0 && y < ny && z >= 0 && z < nz;
-    R.valid |= ok ? (1 << r) : 0;
-    R.rowoff[r] = ((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs;
+      for (int dz = 0; dz < 2; dz++)
 #pragma unroll
-    for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
-    if (ok) load_seg<S>(src + R.rowoff[r], R.own[r]);
-    if (R.m < S) {
-#pragma unroll
-      for (int j = 0; j < ND; j++) R.own[r][j] = keep_bytes(R.own[r][j], R.m - 4 * j);
+        for (int dy = 0; dy < 2; dy++) {
+          const int yo = R.yq + dy - p, zo = R.zq + dz - p;
+          if (yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o) o |= sk->other[(long long)zo * sk->nyq_o + yo];
+        }
+      // segments seg, seg+1 and seg-1 at tick t-1 (shifted right only: seg may be 63)
+      const u64 near = ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u);
+      const u64 own = (sk->mine[q] >> R.seg) & 1u;  // this segment at tick t-2
+      R.skip = (near | own) == 0;
     }
   }
+  rq_fetch<S>(R, src, nx, ny, nz, p, !SKIP || !R.skip || sk->load_skipped);
 }
 
 // Evaluates the blocks of a loaded row quad at tick t and stores the new bytes.
 // Returns this lane's share of the hash of the written cells when HASH.  Every
 // lane of the wave must call it together (DPP exchange).
-template <int S, bool HASH>
-__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule) {
+template <int S, bool HASH, bool SKIP = false>
+__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
+                                         bool *changed = nullptr) {
   constexpr int ND = S / 4;  // dwords per lane per row
   const int p = (int)(t & 1u);
   const int seg = R.seg;
@@ -203,7 +255,8 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
     const u32 from_r = lane_from_right(R.own[r][0]);
     if (p) {
       const u32 lb = seg == 0 ? 0x01000000u : (from_l & 0xFF000000u);
-      const u32 rb = seg == LR - 1 ? 0x01u : (from_r & 0xFFu);
+      // frame bytes 2-3 of the last dword hold no block: STATIC, like the outputs
+      const u32 rb = (seg == LR - 1 ? 0x01u : (from_r & 0xFFu)) | 0x00010100u;
       fr[r][0] = __builtin_amdgcn_alignbyte(R.own[r][0], lb, 3);
 #pragma unroll
       for (int j = 1; j < ND; j++) fr[r][j] = __builtin_amdgcn_alignbyte(R.own[r][j], R.own[r][j - 1], 3);
@@ -214,6 +267,18 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
       fr[r][ND] = kStatic4;
     }
   }
+
+  // ---- neighbours skipped without loading (odd parity) ----
+  // Their edge bytes are placeholders; the block straddling into such a
+  // neighbour is settled (its own bit covered it), so it keeps its input.
+  bool nl = false, nr = false;
+  if constexpr (SKIP) {
+    const u32 unl = R.loaded ? 0u : 1u;
+    const u32 fl = lane_from_left(unl), fr_ = lane_from_right(unl);
+    nl = p && seg > 0 && fl != 0;
+    nr = p && seg < LR - 1 && fr_ != 0;
+  }
+  u32 dif = 0;
 
   // ---- variants: one 32-bit word per 16 consecutive blocks ----
   // A lane evaluates blocks bx0 .. bx0 + S/2 (the last one on odd ticks only);
@@ -256,10 +321,30 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
       e[h] = rule[(v << 16) | s];
     }
     const u32 lo0 = (u32)e[0], hi0 = (u32)(e[0] >> 32), lo1 = (u32)e[1], hi1 = (u32)(e[1] >> 32);
-    out[0][j] = __builtin_amdgcn_perm(lo1, lo0, 0x05040100u);
+    // Entries flag a changing block in bit 7 of cell 0's byte (out[0] bytes 0, 2).
+    bool keep_in = false;
+    if constexpr (SKIP) {
+      keep_in = (j == 0 && nl) || (j == ND && nr);  // block 0 straddles into a settled lane
+      dif |= (keep_in ? lo1 : (lo0 | lo1)) & (u32)kRuleChanged;
+    }
+    out[0][j] = __builtin_amdgcn_perm(lo1, lo0, 0x05040100u) & 0x7F7F7F7Fu;
     out[1][j] = __builtin_amdgcn_perm(lo1, lo0, 0x07060302u);
     out[2][j] = __builtin_amdgcn_perm(hi1, hi0, 0x05040100u);
     out[3][j] = __builtin_amdgcn_perm(hi1, hi0, 0x07060302u);
+    if constexpr (SKIP) {
+      // The settled straddling block keeps its input: of its bytes, only this
+      // lane's own (frame byte 1 at j = 0, frame byte 0 at j = ND) is stored.
+      if (keep_in && j == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) out[r][0] = (out[r][0] & 0xFFFF00FFu) | ((R.own[r][0] & 0xFFu) << 8);
+      } else if (keep_in) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) out[r][ND] = (out[r][ND] & 0xFFFFFF00u) | (R.own[r][ND - 1] >> 24);
+      }
+    }
+  }
+  if constexpr (SKIP) {
+    if (changed) *changed = R.loaded && dif != 0;
   }
 
   // ---- back to own bytes, restore reserved values, store, hash ----
@@ -278,7 +363,9 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
     }
     if (R.valid >> r & 1) {
       u8 *dp = dst + R.rowoff[r];
-      if (R.m == S) {
+      if (SKIP && R.skip) {
+        // settled: the destination already holds these bytes
+      } else if (R.m == S) {
         store_seg<S>(dp, res);
       } else {
 #pragma unroll

@@ -21,6 +21,8 @@ EMPTY, STATIC, SAND, FLUID = 0, 1, 2, 3
 # Scene parameters of the measured workloads.
 BOXES_512 = dict(side_min=8, side_max=64, pct=(20, 10, 5), max_boxes=1 << 20)
 PILLARS_2048 = (256, 4, 32, 256, 1800, 256, 682, 256, 682, 32, 128)
+DENSE = 1  # world flag: settled-segment skipping off
+SKIP_OFF, SKIP_AUTO, SKIP_LISTED, SKIP_WAVES = 0, 1, 2, 3
 
 
 def device_count():
@@ -195,6 +197,27 @@ class World:
     @segment.setter
     def segment(self, s):
         check(lib().fs3d_world_set_segment(self.handle, s))
+
+    @property
+    def skip(self):
+        """Settled-segment skipping mode: SKIP_OFF, SKIP_AUTO (the default unless
+        the world was created with DENSE), SKIP_LISTED or SKIP_WAVES.  Setting
+        True / False means SKIP_AUTO / SKIP_OFF."""
+        s = c_i32(0)
+        check(lib().fs3d_world_get_skip(self.handle, ctypes.byref(s)))
+        return s.value
+
+    @skip.setter
+    def skip(self, mode):
+        check(lib().fs3d_world_set_skip(self.handle, int(mode)))
+
+    def sweep_stats(self, enable=True):
+        """Counters since the previous call: voxels read and written, voxels only
+        read (settled segments on hash ticks), and settled segments that changed
+        (always 0).  enable keeps (or starts) counting; False stops it."""
+        out = (c_u64 * 3)()
+        check(lib().fs3d_world_sweep_stats(self.handle, 1 if enable else 0, out))
+        return dict(swept=int(out[0]), read_only=int(out[1]), violations=int(out[2]))
 
     # -- scenes ----------------------------------------------------------------
     def fill_rain(self, seed, thr):

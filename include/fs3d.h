@@ -67,7 +67,9 @@ int fs3d_rule_lut(uint16_t *lut /* 4*65536 */);
 /* ---- world grid ------------------------------------------------------------ */
 /* uint8 v[z][y][x], x fastest, z up.  No reference counterpart exists (the
  * reference Renderer holds meshes, materials and a camera only,
- * src/engine/rendering/renderer.h:100-154). */
+ * src/engine/rendering/renderer.h:100-154).  flags: FS3D_WORLD_DENSE turns
+ * settled-segment skipping off (see fs3d_world_set_skip). */
+enum { FS3D_WORLD_DENSE = 1 };
 int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out);
 int fs3d_world_destroy(fs3d_world *w); /* NULL is a no-op */
 int fs3d_world_dims(const fs3d_world *w, int32_t dims[3]);
@@ -102,6 +104,29 @@ int fs3d_world_device_ptr(fs3d_world *w, const void **dptr);
 /* Tick kernel launch shape: bytes per lane along x (8, 16, 32 or 64; 0 = auto). */
 int fs3d_world_set_segment(fs3d_world *w, int32_t seg_bytes);
 int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg_bytes);
+
+/* Settled-segment skipping (FS3D_SKIP_AUTO by default).  A tick leaves a lane
+ * segment (S bytes of 4 rows) unread and unwritten when none of the blocks it
+ * evaluates changed at the previous tick of the same parity and none of their
+ * cells changed at the tick before; results are bit-identical to the dense tick
+ * (whether a block is a fixed point does not depend on its variant).  Two
+ * schedules: LISTED runs a plan pass that lists the row quads with work and a
+ * persistent pass over the list (settled regions cost nothing; best for large
+ * grids), WAVES launches one wave per row-quad group that skips on its own
+ * (lower per-tick overhead; best for small grids).  AUTO picks LISTED above
+ * 2^17 waves of work per tick.  Hash ticks read settled segments in either.
+ * Every host-side edit (set_voxel, upload, fills, set_tick, set_segment) keeps
+ * the bookkeeping exact; writing the grid through fs3d_world_device_ptr is not
+ * supported. */
+enum { FS3D_SKIP_OFF = 0, FS3D_SKIP_AUTO = 1, FS3D_SKIP_LISTED = 2, FS3D_SKIP_WAVES = 3 };
+int fs3d_world_set_skip(fs3d_world *w, int32_t mode);
+int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode);
+/* Sweep counters of the skipping tick: enable != 0 starts (or keeps) counting,
+ * 0 stops.  Returns and clears what was counted so far: out[0] = voxels read and
+ * written, out[1] = voxels only read (settled segments on hash ticks), out[2] =
+ * settled segments whose blocks changed on a hash tick (always 0 unless the
+ * skipping bookkeeping is wrong).  Synchronises. */
+int fs3d_world_sweep_stats(fs3d_world *w, int32_t enable, uint64_t out[3]);
 
 /* ---- scenes (seeded, bit-exact, generated by this library) ------------------- */
 /* Rain fill: STATIC floor at z = 0; cells with z >= max(1, nz/2) are SAND when

@@ -57,8 +57,9 @@ SHAPES = [(1, 1, 1), (1, 5, 7), (2, 2, 2), (3, 4, 5), (7, 3, 9), (8, 8, 8), (15,
           (2047, 2, 3), (2048, 3, 3), (4095, 2, 2), (4096, 3, 2)]
 
 
+@pytest.mark.parametrize("dense", [False, True], ids=["skip", "dense"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
-def test_random_worlds_match_oracle(gpu, O, shape):
+def test_random_worlds_match_oracle(gpu, O, shape, dense):
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 1000003 + ny * 1009 + nz)
     g = rng.choice(np.array([0, 0, 0, 1, 2, 2, 3, 3], np.uint8), size=(nz, ny, nx))
@@ -66,8 +67,8 @@ def test_random_worlds_match_oracle(gpu, O, shape):
     t0 = int(rng.integers(0, 1 << 31))
     segs = [s for s in (8, 16, 32, 64) if (nx + s - 1) // s <= 64]
     for seg in segs:
-        announce(f"random world {shape} seg={seg} t0={t0}")
-        with gpu.World(shape) as w:
+        announce(f"random world {shape} seg={seg} t0={t0} dense={dense}")
+        with gpu.World(shape, flags=gpu.DENSE if dense else 0) as w:
             w.segment = seg
             w.upload(g)
             w.tick_count = t0
