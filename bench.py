@@ -3,16 +3,25 @@
 Default run (one JSON line on stdout):
   value    the 512^3 box scene (20% sand, 10% fluid, 5% static boxes over a floor),
            K ticks in one captured HIP graph with a fused state hash every 50
-           ticks, after W warm-up ticks.  A "step" is one tick of the whole world.
-  roofline the dense tick kernel on that world: 2 algorithmic bytes per
-           voxel-update (1 read + 1 write) / mean kernel time (HIP events on the
-           world's stream over a graph of plain ticks), against 8.0 TB/s.
-  extra    pillars2048: the 2048^3 pillar scene's dense tick (HBM-bound; the 512^3
-           pair of buffers nearly fits the 256 MB Infinity Cache); copy: a plain
-           device copy of the same bytes, the practical bandwidth ceiling.
+           ticks, after W warm-up ticks, with the library's default tick (settled
+           segments skipped, results bit-identical to the dense tick).  A "step" is
+           one tick of the whole world; value counts every voxel of every tick.
+  roofline the timed region: algorithmic bytes actually moved (2 per swept
+           voxel-update, 1 per voxel only read on hash ticks; counted by replaying
+           the same ticks with sweep counters) / device time of the timed graph
+           (HIP events on the world's stream), against 8.0 TB/s; "dense" holds the
+           dense tick kernel on the same world (2 bytes per voxel-update).
+  extra    pillars2048: the 2048^3 pillar scene, skipping and dense (the dense
+           kernel is the HBM-bound figure: the 512^3 pair of buffers nearly fits
+           the 256 MB Infinity Cache); pillars2048_render: that scene with a
+           1920x1080 voxel render every 10 ticks; copy: a plain device copy of the
+           same bytes, the practical bandwidth ceiling; batched: 4096 worlds against
+           the one-launch-per-world loop; serving: the continuous-batching server
+           under a Poisson load.
   cpu_baseline  the oracle's tick on this host, one thread, a bounded sample.
 
---only {boxes512,pillars2048,copy} runs one workload (for profiler runs).
+--only {boxes512,pillars2048,copy,batched,serving,render2048} runs one workload
+(for profiler runs) and prints its own JSON line.
 """
 import argparse
 import json
@@ -38,6 +47,11 @@ def roofline(vox, kern_ms):
 
 
 def bench_boxes512(F, steps, warmup, kernel_ticks=200):
+    """The 512^3 box scene with the library's default tick (settled segments are
+    skipped): warm-up ticks, then the timed graph of `steps` ticks with a fused
+    hash every 50.  The same tick sequence is replayed with sweep counters on to
+    get the algorithmic bytes of the timed region (skipping is deterministic), and
+    a dense world runs the plain tick kernel for the dense roofline."""
     n = 512
     with F.World((n, n, n)) as w:
         t0 = time.perf_counter()
@@ -49,26 +63,56 @@ def bench_boxes512(F, steps, warmup, kernel_ticks=200):
         t0 = time.perf_counter()
         hashes = w.run(steps, 50)
         wall = time.perf_counter() - t0
-        log(f"[bench] boxes512: {steps} ticks in {wall * 1e3:.2f} ms")
-        w.run(kernel_ticks, 0)
-        kern_ms = w.last_run_ms_per_tick()
-        return dict(vox=n ** 3, wall=wall, kern_ms=kern_ms, hashes=[int(h) for h in hashes], seg=w.segment)
+        dev_ms = w.last_run_ms_per_tick()
+        log(f"[bench] boxes512: {steps} ticks in {wall * 1e3:.2f} ms ({dev_ms * 1e3:.2f} us/tick on the device)")
+        # replay with counters
+        w.fill_boxes(1)
+        w.tick_count = 0
+        if warmup:
+            w.run(warmup, 0)
+        w.sweep_stats(True)
+        h2 = w.run(steps, 50)
+        st = w.sweep_stats(False)
+        if [int(x) for x in h2] != [int(x) for x in hashes]:
+            raise SystemExit("boxes512: counter replay diverged from the timed run")
+        mode, seg = w.skip, w.segment
+    with F.World((n, n, n), flags=F.DENSE) as d:
+        d.fill_boxes(1)
+        d.run(kernel_ticks, 0)
+        d.run(kernel_ticks, 0)
+        kern_ms = d.last_run_ms_per_tick()
+    return dict(vox=n ** 3, wall=wall, dev_ms=dev_ms, kern_ms=kern_ms, hashes=[int(h) for h in hashes], seg=seg,
+                swept=st["swept"], read_only=st["read_only"], violations=st["violations"], mode=mode)
 
 
 def bench_pillars2048(F, ticks):
+    """The 2048^3 pillar scene: `ticks` plain ticks in one graph with the default
+    (skipping) tick and with the dense kernel (the HBM roofline figure)."""
     n = 2048
-    with F.World((n, n, n)) as w:
-        t0 = time.perf_counter()
-        w.fill_pillars(2)
-        log(f"[bench] pillars2048: scene generated in {time.perf_counter() - t0:.2f}s")
-        w.run(4, 0)
-        t0 = time.perf_counter()
-        w.run(ticks, 0)
-        wall = time.perf_counter() - t0
-        kern_ms = w.last_run_ms_per_tick()
-        log(f"[bench] pillars2048: {ticks} ticks in {wall * 1e3:.1f} ms ({kern_ms:.3f} ms/tick)")
-        h = w.hash()
-        return dict(vox=n ** 3, wall=wall, kern_ms=kern_ms, ticks=ticks, hash=int(h), seg=w.segment)
+    out = {}
+    for dense in (False, True):
+        with F.World((n, n, n), flags=F.DENSE if dense else 0) as w:
+            t0 = time.perf_counter()
+            w.fill_pillars(2)
+            log(f"[bench] pillars2048: scene generated in {time.perf_counter() - t0:.2f}s")
+            w.run(4, 0)
+            w.fill_pillars(2)
+            w.tick_count = 0
+            if not dense:
+                w.sweep_stats(True)
+            t0 = time.perf_counter()
+            w.run(ticks, 0)
+            wall = time.perf_counter() - t0
+            kern_ms = w.last_run_ms_per_tick()
+            st = w.sweep_stats(False) if not dense else None
+            h = w.hash()
+            log(f"[bench] pillars2048 {'dense' if dense else 'skip'}: {ticks} ticks in {wall * 1e3:.1f} ms "
+                f"({kern_ms:.3f} ms/tick)")
+            out["dense" if dense else "skip"] = dict(wall=wall, kern_ms=kern_ms, hash=int(h),
+                                                     swept=st["swept"] if st else None, seg=w.segment)
+    if out["dense"]["hash"] != out["skip"]["hash"]:
+        raise SystemExit("pillars2048: skipping and dense ticks disagree")
+    return dict(vox=n ** 3, ticks=ticks, **out)
 
 
 def orbit_angles(seed, n):
@@ -149,6 +193,19 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
         wall = time.perf_counter() - t0
         dev_ms = b.last_run_ms()
         log(f"[bench] batched: persistent {ticks} ticks in {wall * 1e3:.1f} ms (device {dev_ms:.1f} ms)")
+        # the same run with every tick of every world executed (settled worlds too)
+        b.skip = False
+        b.tick_count = 0
+        b.fill_rain(seeds, thr)
+        b.run(ticks)
+        b.tick_count = 0
+        b.fill_rain(seeds, thr)
+        t0 = time.perf_counter()
+        h_all = b.run(ticks)
+        wall_all = time.perf_counter() - t0
+        b.skip = True
+        exact = exact and bool((h_all == h).all())
+        log(f"[bench] batched: every tick executed: {wall_all * 1e3:.1f} ms, same hashes={bool((h_all == h).all())}")
         digest = 0
         for v in h:
             digest = (digest + int(v)) % (1 << 64)
@@ -157,6 +214,8 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
                     gvox_per_s=vox * ticks / wall / 1e9, plain_ms_per_tick=plain_per_tick * 1e3,
                     plain_gvox_per_s=vox / plain_per_tick / 1e9,
                     speedup=(plain_per_tick * ticks) / wall, bit_exact_sample=exact,
+                    all_ticks_wall=wall_all, all_ticks_gvox_per_s=vox * ticks / wall_all / 1e9,
+                    all_ticks_speedup=(plain_per_tick * ticks) / wall_all,
                     tuned={"mode": {0: "persistent", 2: "wide"}.get(tune["mode"], tune["mode"]), "seg": tune["seg"],
                            "threads": tune["threads"], "groups": tune["groups"]}, hash_digest=digest)
 
@@ -226,7 +285,8 @@ def main():
 
     if args.only == "pillars2048":
         p = bench_pillars2048(F, args.steps)
-        print(json.dumps({"workload": "pillars2048", "kernel_ms": p["kern_ms"], "hash": p["hash"]}), flush=True)
+        print(json.dumps({"workload": "pillars2048", "dense_kernel_ms": p["dense"]["kern_ms"],
+                          "skip_ms_per_tick": p["skip"]["kern_ms"], "hash": p["dense"]["hash"]}), flush=True)
         return
     if args.only == "copy":
         c = bench_copy(F, 8 << 30)
@@ -249,7 +309,13 @@ def main():
     r = bench_boxes512(F, args.steps, args.warmup)
     vox = r["vox"]
     value = vox * args.steps / r["wall"] / 1e9
-    ach, frac = roofline(vox, r["kern_ms"])
+    # the timed region: algorithmic bytes actually moved (2 per swept voxel-update,
+    # 1 per voxel only read on hash ticks) over its device time (HIP events)
+    alg_bytes = 2 * r["swept"] + r["read_only"]
+    ach = alg_bytes / (r["dev_ms"] * 1e-3 * args.steps)
+    frac = ach / HBM_PEAK
+    ach_d, frac_d = roofline(vox, r["kern_ms"])
+    sched = {1: "auto", 2: "listed", 3: "waves"}.get(r["mode"], str(r["mode"]))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -268,13 +334,20 @@ def main():
                    "dims": [512, 512, 512], "hash_every": 50, "segment_bytes": r["seg"]},
         "roofline": {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(frac, 4), "traffic": None,
-                     "kernel": f"tick_kernel<{r['seg']},false>", "kernel_ms": round(r["kern_ms"], 5)},
+                     "kernel": f"tick_kernel<{r['seg']},*,skip> ({sched} schedule), every tick of the timed region",
+                     "ms_per_tick_device": round(r["dev_ms"], 5),
+                     "alg_bytes_per_tick": int(alg_bytes / args.steps),
+                     "swept_fraction": round(r["swept"] / (vox * args.steps), 4),
+                     "skip_violations": r["violations"],
+                     "dense": {"kernel": f"tick_kernel<{r['seg']},false> (every voxel, no skipping)",
+                               "kernel_ms": round(r["kern_ms"], 5), "achieved": round(ach_d / 1e9, 1),
+                               "frac": round(frac_d, 4)}},
         "final_hash": r["hashes"][-1] if r["hashes"] else None,
     }
     if args.only is None:
         extra = {}
         p = bench_pillars2048(F, 200)
-        ach2, frac2 = roofline(p["vox"], p["kern_ms"])
+        ach2, frac2 = roofline(p["vox"], p["dense"]["kern_ms"])
         c = bench_copy(F, 8 << 30)
         pr = bench_pillars_render(F)
         extra["pillars2048_render"] = {
@@ -284,12 +357,15 @@ def main():
             "render_ms_mean": round(pr["render_ms"], 3), "render_ms_max": round(pr["render_ms_max"], 3),
             "rays_per_s": round(pr["rays"] / (pr["render_ms"] * 1e-3), 1), "hit_fraction": round(pr["hit_frac"], 4),
             "final_hash": pr["hash"]}
+        ps, pd = p["skip"], p["dense"]
         extra["pillars2048"] = {
             "workload": "2048^3 uint8, 256 seeded sand pillars in a fluid pool, 200 plain ticks in one graph",
-            "gvox_per_s": round(p["vox"] * p["ticks"] / p["wall"] / 1e9, 2),
-            "kernel_ms": round(p["kern_ms"], 4), "achieved_GBps": round(ach2 / 1e9, 1),
-            "hbm_frac": round(frac2, 4), "final_hash": p["hash"], "segment_bytes": p["seg"],
-            "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
+            "gvox_per_s": round(p["vox"] * p["ticks"] / ps["wall"] / 1e9, 2),
+            "ms_per_tick": round(ps["kern_ms"], 4), "swept_fraction": round(ps["swept"] / (p["vox"] * p["ticks"]), 4),
+            "dense_gvox_per_s": round(p["vox"] * p["ticks"] / pd["wall"] / 1e9, 2),
+            "dense_kernel_ms": round(pd["kern_ms"], 4), "dense_achieved_GBps": round(ach2 / 1e9, 1),
+            "dense_hbm_frac": round(frac2, 4), "dense_frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
+            "final_hash": pd["hash"], "segment_bytes": pd["seg"]}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4)}
         bb = bench_batched(F)
@@ -299,6 +375,11 @@ def main():
             "device_ms": round(bb["dev_ms"], 2), "plain_gvox_per_s": round(bb["plain_gvox_per_s"], 3),
             "plain_ms_per_tick_sampled": round(bb["plain_ms_per_tick"], 3),
             "speedup_vs_plain": round(bb["speedup"], 2), "bit_exact_sample": bb["bit_exact_sample"],
+            "every_tick_executed": {"gvox_per_s": round(bb["all_ticks_gvox_per_s"], 2),
+                                    "wall_ms": round(bb["all_ticks_wall"] * 1e3, 2),
+                                    "speedup_vs_plain": round(bb["all_ticks_speedup"], 2)},
+            "settled_worlds": "a world with two quiet ticks in a row is a fixed point; its remaining ticks "
+                              "are not run (only its final, hashed tick)",
             "autotuned": bb["tuned"], "final_hash_digest": bb["hash_digest"]}
         sv = bench_serving(F)
         st = sv["stats"]

@@ -109,6 +109,8 @@ SIGNATURES = {
     "fs3d_batch_set_tick": (c_int, [vp, c_u64]),
     "fs3d_batch_set_config": (c_int, [vp, c_i32, c_i32, c_i32]),
     "fs3d_batch_get_config": (c_int, [vp, P(c_i32), P(c_i32), P(c_i32), P(c_i32)]),
+    "fs3d_batch_set_skip": (c_int, [vp, c_i32]),
+    "fs3d_batch_get_skip": (c_int, [vp, P(c_i32)]),
     "fs3d_batch_run": (c_int, [vp, c_i32, c_i32, P(c_u64)]),
     "fs3d_batch_last_run_ms": (c_int, [vp, P(c_double)]),
     "fs3d_batch_autotune": (c_int, [vp, c_i32, P(c_i32), P(c_i32), P(c_i32), P(c_i32), P(c_double)]),

@@ -108,6 +108,17 @@ class Batch:
     def set_config(self, seg, threads, groups=1):
         check(lib().fs3d_batch_set_config(self.handle, seg, threads, groups))
 
+    @property
+    def skip(self):
+        """Settled worlds stop ticking in persistent runs (results are identical)."""
+        on = c_i32(0)
+        check(lib().fs3d_batch_get_skip(self.handle, ctypes.byref(on)))
+        return bool(on.value)
+
+    @skip.setter
+    def skip(self, on):
+        check(lib().fs3d_batch_set_skip(self.handle, 1 if on else 0))
+
     def run(self, nticks, plain=False, hashes=True, mode=None):
         """Advances every world; returns the per-world hashes after the last tick.
         mode: PERSISTENT, WIDE, PLAIN or TUNED (default: TUNED, or PLAIN when

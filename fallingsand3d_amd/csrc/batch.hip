@@ -24,8 +24,9 @@ using namespace tickcore;
 
 template <int S, int NT, int NR>
 __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *order, int n, int nticks, u32 *queue,
-                                                   const u64 *__restrict__ rule, u64 *hashes, int want_hash) {
+                                                   const u64 *__restrict__ rule, u64 *hashes, int flags) {
   constexpr int NW = NT / 64;
+  const bool want_hash = flags & kBatchHash, settle = flags & kBatchSettle;
   __shared__ int s_idx;
   __shared__ u64 s_sum[NW];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -43,20 +44,29 @@ __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *or
     const int wi = order[idx];
     const BWorld w = worlds[wi];
     u64 acc = 0;
+    int quiet = 0;  // consecutive ticks in which no cell of the world changed
     for (int k = 0; k < nticks; k++) {
+      // Two quiet ticks in a row (one of each parity) make the state a fixed
+      // point of both parities, whatever the variants: the remaining ticks
+      // change nothing, so only the last (hashed) one is run.
+      if (settle && quiet >= 2) {
+        if (!want_hash) break;
+        k = nticks - 1;
+      }
       const u32 t = (u32)(w.tick + (u64)k);
       const int p = (int)(t & 1u);
       const long long nrq = (long long)((w.ny + p + 1) >> 1) * ((w.nz + p + 1) >> 1);
       const long long nwav = (nrq + w.rqpw - 1) / w.rqpw;
+      bool ch = false;
       // each wave takes NR row-quad groups per step (wv, wv + NW, ...)
       if (want_hash && k == nticks - 1) {
         for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
-          acc += tick_waves<S, true, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule);
+          acc += tick_waves<S, true, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
       } else {
         for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
-          tick_waves<S, false, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule);
+          tick_waves<S, false, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
       }
-      __syncthreads();
+      quiet = __syncthreads_or(ch) ? 0 : quiet + 1;
     }
     if (want_hash) {
       acc = wave_sum_u64(acc);
@@ -121,9 +131,9 @@ __global__ void reset_u32_kernel(u32 *p) {
 
 template <int S, int NT, int NR>
 hipError_t launch_batch_t(BWorld *worlds, const int *order, int n, int nticks, u32 *queue, const u64 *rule,
-                          u64 *hashes, int want_hash, int grid, hipStream_t s) {
+                          u64 *hashes, int flags, int grid, hipStream_t s) {
   hipLaunchKernelGGL((batch_kernel<S, NT, NR>), dim3(grid), dim3(NT), 0, s, worlds, order, n, nticks, queue, rule,
-                     hashes, want_hash);
+                     hashes, flags);
   return hipGetLastError();
 }
 
@@ -175,12 +185,12 @@ int batch_occupancy(int seg, int threads, int groups) {
 }
 
 hipError_t launch_batch(int seg, int threads, int groups, BWorld *worlds, const int *order, int n, int nticks,
-                        u32 *queue, const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s) {
+                        u32 *queue, const u64 *rule, u64 *hashes, int flags, int grid, hipStream_t s) {
   hipLaunchKernelGGL(reset_u32_kernel, dim3(1), dim3(64), 0, s, queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
 #define LB(S_, N_, R_) \
-  return launch_batch_t<S_, N_, R_>(worlds, order, n, nticks, queue, rule, hashes, want_hash, grid, s)
+  return launch_batch_t<S_, N_, R_>(worlds, order, n, nticks, queue, rule, hashes, flags, grid, s)
   FS3D_BATCH_DISPATCH(seg, threads, groups, LB)
 #undef LB
 }

@@ -34,6 +34,7 @@ struct fs3d_batch {
   std::vector<BWorld> h_worlds;
   u64 tick = 0;  // all worlds of a batch share the tick counter
   int seg = 16, threads = 256, groups = 1, grid = 0;
+  bool settle = true;  // persistent mode: stop ticking worlds that have settled
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -133,13 +134,14 @@ void release(fs3d_batch *b) {
 }
 
 int run_persistent(fs3d_batch *b, int nticks, int want_hash) {
-  const auto key = std::make_tuple((int)FS3D_BATCH_PERSISTENT, nticks, want_hash, 0, b->seg, b->threads, b->groups);
+  const int flags = (want_hash ? kBatchHash : 0) | (b->settle ? kBatchSettle : 0);
+  const auto key = std::make_tuple((int)FS3D_BATCH_PERSISTENT, nticks, flags, 0, b->seg, b->threads, b->groups);
   auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
     hipGraph_t graph = nullptr;
     FS3D_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_batch(b->seg, b->threads, b->groups, b->d_worlds, b->d_order, b->n, nticks, b->d_queue,
-                                device_rule_table(b->dev), b->d_hashes, want_hash, b->grid, b->stream);
+                                device_rule_table(b->dev), b->d_hashes, flags, b->grid, b->stream);
     hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
     if (e != hipSuccess || e2 != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -361,6 +363,19 @@ int fs3d_batch_get_config(const fs3d_batch *b, int32_t *seg, int32_t *threads, i
   if (threads) *threads = b->threads;
   if (groups) *groups = b->groups;
   if (grid) *grid = b->grid;
+  return clear_error();
+}
+
+int fs3d_batch_set_skip(fs3d_batch *b, int32_t on) {
+  if (int rc = check_batch(b)) return rc;
+  b->settle = on != 0;
+  return clear_error();
+}
+
+int fs3d_batch_get_skip(const fs3d_batch *b, int32_t *on) {
+  if (int rc = check_batch(b)) return rc;
+  if (!on) return set_error(FS3D_ERR_INVALID, "on is NULL");
+  *on = b->settle ? 1 : 0;
   return clear_error();
 }
 

@@ -18,8 +18,12 @@ struct BWorld {
 
 bool batch_config_ok(int seg, int threads, int groups);
 int batch_occupancy(int seg, int threads, int groups);
+// flags of the persistent batch launch: hash every world's final tick; stop
+// ticking a world after two ticks in which nothing changed (its state is then a
+// fixed point of both parities) and run only its final tick.
+constexpr int kBatchHash = 1, kBatchSettle = 2;
 hipError_t launch_batch(int seg, int threads, int groups, BWorld *worlds, const int *order, int n, int nticks,
-                        u32 *queue, const u64 *rule, u64 *hashes, int want_hash, int grid, hipStream_t s);
+                        u32 *queue, const u64 *rule, u64 *hashes, int flags, int grid, hipStream_t s);
 hipError_t launch_batch_hash(const BWorld *worlds, int n, u64 *hashes, hipStream_t s);
 hipError_t launch_batch_fill_rain(const BWorld *worlds, int n, const u64 *seeds, const u64 *thr, size_t max_cells,
                                   hipStream_t s);

@@ -231,7 +231,9 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // Evaluates the blocks of a loaded row quad at tick t and stores the new bytes.
 // Returns this lane's share of the hash of the written cells when HASH.  Every
 // lane of the wave must call it together (DPP exchange).
-template <int S, bool HASH, bool SKIP = false>
+// With SKIP or TRACK, *changed reports whether a block this lane evaluated
+// changed.
+template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP>
 __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
                                          bool *changed = nullptr) {
   constexpr int ND = S / 4;  // dwords per lane per row
@@ -323,10 +325,8 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
     const u32 lo0 = (u32)e[0], hi0 = (u32)(e[0] >> 32), lo1 = (u32)e[1], hi1 = (u32)(e[1] >> 32);
     // Entries flag a changing block in bit 7 of cell 0's byte (out[0] bytes 0, 2).
     bool keep_in = false;
-    if constexpr (SKIP) {
-      keep_in = (j == 0 && nl) || (j == ND && nr);  // block 0 straddles into a settled lane
-      dif |= (keep_in ? lo1 : (lo0 | lo1)) & (u32)kRuleChanged;
-    }
+    if constexpr (SKIP) keep_in = (j == 0 && nl) || (j == ND && nr);  // block 0 straddles into a settled lane
+    if constexpr (SKIP || TRACK) dif |= (keep_in ? lo1 : (lo0 | lo1)) & (u32)kRuleChanged;
     out[0][j] = __builtin_amdgcn_perm(lo1, lo0, 0x05040100u) & 0x7F7F7F7Fu;
     out[1][j] = __builtin_amdgcn_perm(lo1, lo0, 0x07060302u);
     out[2][j] = __builtin_amdgcn_perm(hi1, hi0, 0x05040100u);
@@ -343,7 +343,7 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
       }
     }
   }
-  if constexpr (SKIP) {
+  if constexpr (SKIP || TRACK) {
     if (changed) *changed = R.loaded && dif != 0;
   }
 
@@ -398,16 +398,22 @@ __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny,
 // issued before the first update, so NR groups' memory latency overlaps.  Row
 // quads are disjoint, so loading ahead is safe also in place.  Groups past the
 // end of the tick are inactive (no loads or stores).
+// With `changed`, ORs into *changed whether any block of this lane changed.
 template <int S, bool HASH, int NR>
 __device__ __forceinline__ u64 tick_waves(const u8 *src, u8 *dst, int nx, int ny, int nz, int LR, int rqpw, u32 t,
-                                          long long wave0, long long stride, int lane, const u64 *__restrict__ rule) {
+                                          long long wave0, long long stride, int lane, const u64 *__restrict__ rule,
+                                          bool *changed = nullptr) {
   RowQuad<S> R[NR];
   const int p = (int)(t & 1u);
 #pragma unroll
   for (int i = 0; i < NR; i++) rq_load<S>(R[i], src, nx, ny, nz, LR, rqpw, p, wave0 + i * stride, lane);
   u64 acc = 0;
 #pragma unroll
-  for (int i = 0; i < NR; i++) acc += rq_update<S, HASH>(R[i], dst, LR, t, rule);
+  for (int i = 0; i < NR; i++) {
+    bool ch = false;
+    acc += rq_update<S, HASH, false, true>(R[i], dst, LR, t, rule, &ch);
+    if (changed) *changed = *changed || ch;
+  }
   return acc;
 }
 

@@ -183,6 +183,12 @@ int fs3d_batch_set_tick(fs3d_batch *b, uint64_t tick);
  * number of resident workgroups. */
 int fs3d_batch_set_config(fs3d_batch *b, int32_t seg, int32_t threads, int32_t groups);
 int fs3d_batch_get_config(const fs3d_batch *b, int32_t *seg, int32_t *threads, int32_t *groups, int32_t *grid);
+/* Settled worlds (on by default): in FS3D_BATCH_PERSISTENT runs a world in which
+ * no cell changed for two consecutive ticks is a fixed point of both parities,
+ * so its remaining ticks are not run (only its final, hashed tick is); results
+ * are bit-identical either way. */
+int fs3d_batch_set_skip(fs3d_batch *b, int32_t on);
+int fs3d_batch_get_skip(const fs3d_batch *b, int32_t *on);
 /* Advances every world by nticks; when hashes != NULL writes each world's state
  * hash after the last tick (hashes[n]).  Synchronises. */
 int fs3d_batch_run(fs3d_batch *b, int32_t nticks, int32_t mode, uint64_t *hashes);
