@@ -110,6 +110,7 @@ void release(fs3d_world *w) {
   if (w->d_rgb) (void)hipFree(w->d_rgb);
   if (w->d_hit) (void)hipFree(w->d_hit);
   if (w->d_face) (void)hipFree(w->d_face);
+  if (w->d_occ) (void)hipFree(w->d_occ);
   if (w->d_small) (void)hipFree(w->d_small);
   for (int i = 0; i < 2; i++)
     if (w->d_masks[i]) (void)hipFree(w->d_masks[i]);

@@ -36,8 +36,9 @@ __host__ __device__ inline long long world_waves_dev(int ny, int nz, int rqpw, i
 
 // Wide launch k of a round: entry i owns workgroups [bpre[i], bpre[i+1]) (4 waves
 // each; none when it sits the launch out); blk2ent maps workgroup -> entry.
+// chg (optional): settled-world flags, four u32 per hash slot (see wide.hip).
 hipError_t launch_wide(int seg, const BWorld *ents, const long long *bpre, const int *blk2ent, long long blocks, int k,
-                       const u64 *rule, u64 *hashes, hipStream_t s);
+                       const u64 *rule, u64 *hashes, hipStream_t s, u32 *chg = nullptr);
 hipError_t launch_expand(const long long *bpre, int n, int *blk2ent, hipStream_t s);
 hipError_t launch_zero_slots(u64 *hashes, const BWorld *ents, int n, hipStream_t s);
 long long world_waves(const BWorld &w, int p);

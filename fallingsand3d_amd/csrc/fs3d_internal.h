@@ -141,6 +141,7 @@ struct fs3d_world {
   fs3d::u64 *d_hit = nullptr;
   fs3d::u8 *d_face = nullptr;
   size_t render_px = 0;
+  fs3d::u64 *d_occ = nullptr;  // brick occupancy map of the renderer
   double last_render_ms = 0.0;
   // settled-segment skipping: mode (FS3D_SKIP_*) and per-parity change masks
   // (one u64 per row quad)

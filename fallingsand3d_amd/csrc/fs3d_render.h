@@ -16,8 +16,11 @@ struct RenderCam {
 };
 
 hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float *d_rgba, hipStream_t s);
+// Brick occupancy words of a grid (8x8x8 bricks, 64 bricks per word along x).
+size_t occ_words(int nx, int ny, int nz);
+// Builds the occupancy map in d_occ (occ_words u64), then casts the rays.
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
-                      u64 *d_hit, u8 *d_face, hipStream_t s);
+                      u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s);
 void palette_host(float *rgba);
 
 }  // namespace fs3d

@@ -32,6 +32,9 @@ __constant__ u32 kPaletteBits[1024] = {
 #include "palette.inc"
 };
 
+typedef u32 u32x2 __attribute__((ext_vector_type(2)));
+typedef u32x2 __attribute__((aligned(1))) u32x2_u;  // rows start at any byte
+
 struct F3 {
   float x, y, z;
 };
@@ -93,8 +96,47 @@ __device__ __forceinline__ int palette_index(u32 m) {
   return m == FS3D_STATIC ? 127 : m == FS3D_SAND ? 162 : m == FS3D_FLUID ? 137 : (int)m;
 }
 
+// Brick occupancy of a grid: one bit per 8x8x8 brick (set when any cell is not
+// EMPTY), occ[(bz * bny + by) * bwords + bx / 64] bit bx % 64.  One wave covers 64
+// bricks along x of one brick row: each lane ORs the 64 rows of 8 bytes of its
+// brick (loads coalesced across the wave), bytes past nx masked off.
+__global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, int bnx, int bny,
+                                                  int bnz, int bwords, u64 *occ) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long nwords = (long long)bwords * bny * bnz;
+  if (wave >= nwords) return;
+  const int wx = (int)(wave % bwords);
+  const long long rest = wave / bwords;
+  const int by = (int)(rest % bny), bz = (int)(rest / bny);
+  const int bx = wx * 64 + lane;
+  u64 acc = 0;
+  if (bx < bnx) {
+    const int x0 = bx * 8;
+    const int keep = min(8, nx - x0);
+    const u64 mask = keep >= 8 ? ~0ull : ((1ull << (8 * keep)) - 1ull);
+    for (int dz = 0; dz < 8; dz++) {
+      const int z = bz * 8 + dz;
+      if (z >= nz) break;
+      u32x2 v[8];
+#pragma unroll
+      for (int dy = 0; dy < 8; dy++) {  // eight loads in flight, rows past ny clamped
+        const int y = min(by * 8 + dy, ny - 1);
+        // unaligned 8-byte load; at most 7 bytes past the row end, inside the
+        // next row or the buffer's guard bytes, and masked off
+        v[dy] = *reinterpret_cast<const u32x2_u *>(g + ((size_t)z * ny + y) * nx + x0);
+      }
+#pragma unroll
+      for (int dy = 0; dy < 8; dy++) acc |= (((u64)v[dy].y << 32) | v[dy].x) & mask;
+    }
+  }
+  const u64 bits = __ballot(acc != 0);
+  if (lane == 0) occ[wave] = bits;
+}
+
 __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, RenderCam cam,
-                                                  int W, int H, float *rgb, u64 *hit, u8 *face) {
+                                                  int W, int H, float *rgb, u64 *hit, u8 *face,
+                                                  const u64 *__restrict__ occ, int bny, int bwords) {
   const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= W || py >= H) return;
   const size_t pix = (size_t)py * W + px;
@@ -154,9 +196,19 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
     int f = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
     float t = tmin;
     const int maxit = nx + ny + nz + 3;
+    // Inside a brick whose occupancy bit is clear every cell is EMPTY: the walk
+    // steps through it exactly as cell by cell, without loading the cells.
+    long long cur_brick = -1;
+    bool brick_empty = false;
     for (int it = 0; it < maxit; it++) {
       const size_t li = ((size_t)idx[2] * ny + idx[1]) * nx + idx[0];
-      const u32 m = g[li];
+      const long long bw = ((long long)(idx[2] >> 3) * bny + (idx[1] >> 3)) * bwords + (idx[0] >> 9);
+      const long long brick = bw * 64 + ((idx[0] >> 3) & 63);
+      if (brick != cur_brick) {
+        cur_brick = brick;
+        brick_empty = ((occ[bw] >> ((idx[0] >> 3) & 63)) & 1ull) == 0;
+      }
+      const u32 m = brick_empty ? 0u : g[li];
       if (m != 0) {
         const int ci = palette_index(m);
         F3 n{0.0f, 0.0f, 0.0f};
@@ -208,10 +260,20 @@ hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float 
   return hipGetLastError();
 }
 
+size_t occ_words(int nx, int ny, int nz) {
+  const size_t bwords = ((size_t)(nx + 7) / 8 + 63) / 64;
+  return bwords * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
+}
+
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
-                      u64 *d_hit, u8 *d_face, hipStream_t s) {
+                      u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s) {
+  const int bnx = (nx + 7) / 8, bny = (ny + 7) / 8, bnz = (nz + 7) / 8, bwords = (bnx + 63) / 64;
+  const long long waves = (long long)bwords * bny * bnz;
+  hipLaunchKernelGGL(occ_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g, nx, ny, nz, bnx, bny, bnz,
+                     bwords, d_occ);
   const dim3 grid((W + 15) / 16, (H + 15) / 16);
-  hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face);
+  hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face, d_occ, bny,
+                     bwords);
   return hipGetLastError();
 }
 

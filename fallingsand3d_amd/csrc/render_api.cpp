@@ -236,13 +236,15 @@ int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t 
     FS3D_HIP(hipMalloc(&w->d_face, px));
     w->render_px = px;
   }
+  if (!w->d_occ) FS3D_HIP(hipMalloc(&w->d_occ, occ_words(w->nx, w->ny, w->nz) * sizeof(u64)));
   RenderCam rc_cam;
   std::memcpy(rc_cam.origin, cam->origin, sizeof rc_cam.origin);
   std::memcpy(rc_cam.basis, cam->basis, sizeof rc_cam.basis);
   std::memcpy(rc_cam.light, cam->light, sizeof rc_cam.light);
   rc_cam.aspect = cam->aspect;
   FS3D_HIP(hipEventRecord(w->ev0, w->stream));
-  FS3D_HIP(launch_dda(w->buf[w->cur], w->nx, w->ny, w->nz, rc_cam, W, H, w->d_rgb, w->d_hit, w->d_face, w->stream));
+  FS3D_HIP(launch_dda(w->buf[w->cur], w->nx, w->ny, w->nz, rc_cam, W, H, w->d_rgb, w->d_hit, w->d_face, w->d_occ,
+                      w->stream));
   FS3D_HIP(hipEventRecord(w->ev1, w->stream));
   FS3D_HIP(hipMemcpyAsync(rgb, w->d_rgb, px * 3 * sizeof(float), hipMemcpyDeviceToHost, w->stream));
   FS3D_HIP(hipMemcpyAsync(hit, w->d_hit, px * sizeof(u64), hipMemcpyDeviceToHost, w->stream));

@@ -104,6 +104,7 @@ struct fs3d_server {
   size_t used = 0, peak = 0;
   // slots and hashes
   u64 *d_hashes = nullptr;
+  u32 *d_chg = nullptr;  // settled-world flags, four per slot (tick mod 4)
   std::vector<int> free_slots;
   // requests
   std::vector<Request> reqs;
@@ -237,6 +238,8 @@ int admit(fs3d_server *s) {
     if (int rc = alloc_grid(s, r)) return rc;
     FS3D_HIP(launch_fill_rain(r.g, r.nx, r.ny, r.nz, r.seed, r.thr, s->s_fill));
     FS3D_HIP(hipMemsetAsync(s->d_hashes + r.slot, 0, sizeof(u64), s->s_fill));
+    // all four change flags set: the first ticks of a new world always run
+    FS3D_HIP(hipMemsetAsync(s->d_chg + (size_t)r.slot * 4, 1, 4 * sizeof(u32), s->s_fill));
     r.status = kStatusActive;
     r.t_admit = now_s() - s->t0;
     r.remaining = r.ticks;
@@ -313,7 +316,7 @@ int launch_round(fs3d_server *s) {
   FS3D_HIP(hipEventRecord(rd.ev[0], s->s_tick));
   for (int k = 0; k < K; k++) {
     FS3D_HIP(launch_wide(s->seg, d_ents, d_pre + (size_t)k * (n + 1), rd.d_blk + boff, tot[k], k, rule, s->d_hashes,
-                         s->s_tick));
+                         s->s_tick, s->d_chg));
     FS3D_HIP(hipEventRecord(rd.ev[k + 1], s->s_tick));
     boff += (size_t)tot[k];
   }
@@ -375,6 +378,7 @@ int fs3d_server_create(const fs3d_server_config *cfg, fs3d_server **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_fill, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fill_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&s->d_hashes, sizeof(u64) * (size_t)cfg->max_active);
+  if (e == hipSuccess) e = hipMalloc(&s->d_chg, 4 * sizeof(u32) * (size_t)cfg->max_active);
   if (e != hipSuccess) {
     fs3d_server_destroy(s);
     return set_error(FS3D_ERR_HIP, "server init failed: %s", hipGetErrorString(e));
@@ -400,6 +404,7 @@ int fs3d_server_destroy(fs3d_server *s) {
   }
   for (u8 *c : s->chunks) (void)hipFree(c);
   if (s->d_hashes) (void)hipFree(s->d_hashes);
+  if (s->d_chg) (void)hipFree(s->d_chg);
   if (s->fill_ev) (void)hipEventDestroy(s->fill_ev);
   if (s->s_tick) (void)hipStreamDestroy(s->s_tick);
   if (s->s_fill) (void)hipStreamDestroy(s->s_fill);

@@ -386,12 +386,14 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
 
 // Processes row-quad group `wave` of one tick at tick counter t (src may equal
 // dst).  Every lane of the wave must call it.
-template <int S, bool HASH>
+// With TRACK, *changed reports whether a block of this lane changed.
+template <int S, bool HASH, bool TRACK = false>
 __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny, int nz, int LR, int rqpw, u32 t,
-                                         long long wave, int lane, const u64 *__restrict__ rule) {
+                                         long long wave, int lane, const u64 *__restrict__ rule,
+                                         bool *changed = nullptr) {
   RowQuad<S> R;
   rq_load<S>(R, src, nx, ny, nz, LR, rqpw, (int)(t & 1u), wave, lane);
-  return rq_update<S, HASH>(R, dst, LR, t, rule);
+  return rq_update<S, HASH, false, TRACK>(R, dst, LR, t, rule, changed);
 }
 
 // Processes NR row-quad groups (wave0, wave0 + stride, ...) with every load

@@ -20,24 +20,38 @@ namespace {
 
 using namespace tickcore;
 
+// chg (optional): per hash slot, four change flags indexed by tick mod 4.  A
+// launch at tick t reads the flags of ticks t-1 and t-2; when both are clear the
+// world is a fixed point of both parities and the tick is not run, unless it is
+// the world's final, hashed one.  Otherwise the tick runs and sets flag t when a
+// cell changed.  The first workgroup of each entry clears flag t+1, which no wave
+// of this launch reads or writes.  Admission sets all four flags.
 template <int S>
 __global__ __launch_bounds__(256) void wide_kernel(const BWorld *__restrict__ ents, const long long *__restrict__ bpre,
                                                    const int *__restrict__ blk2ent, int k, const u64 *__restrict__ rule,
-                                                   u64 *hashes) {
+                                                   u64 *hashes, u32 *chg) {
   const int e = __builtin_amdgcn_readfirstlane(blk2ent[blockIdx.x]);
   const BWorld w = ents[e];
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long long wave = ((long long)blockIdx.x - bpre[e]) * 4 + wid;
   const u32 t = (u32)(w.tick + (u64)k);
+  const bool final_hash = k + 1 == w.nt && w.fin && hashes;
+  u32 *flags = chg ? chg + (size_t)w.slot * 4 : nullptr;
+  if (flags) {
+    if (wave == 0 && threadIdx.x == 0) flags[(t + 1u) & 3u] = 0;
+    if (!final_hash && flags[(t - 1u) & 3u] == 0 && flags[(t - 2u) & 3u] == 0) return;  // settled world
+  }
   if (wave >= world_waves_dev(w.ny, w.nz, w.rqpw, (int)(t & 1u))) return;  // wave-uniform; no barriers below
   const int lane = threadIdx.x & 63;
-  if (k + 1 == w.nt && w.fin && hashes) {
-    u64 acc = tick_wave<S, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule);
+  bool ch = false;
+  if (final_hash) {
+    u64 acc = tick_wave<S, true, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
     acc = wave_sum_u64(acc);
     if (lane == 0 && acc) atomicAdd((unsigned long long *)(hashes + w.slot), (unsigned long long)acc);
   } else {
-    tick_wave<S, false>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule);
+    tick_wave<S, false, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
   }
+  if (flags && __ballot(ch) && lane == 0) flags[t & 3u] = 1;
 }
 
 // blk2ent[b] = i for every workgroup b of entry i.
@@ -55,13 +69,17 @@ __global__ void zero_slots_kernel(u64 *hashes, const BWorld *__restrict__ ents, 
 }  // namespace
 
 hipError_t launch_wide(int seg, const BWorld *ents, const long long *bpre, const int *blk2ent, long long blocks, int k,
-                       const u64 *rule, u64 *hashes, hipStream_t s) {
+                       const u64 *rule, u64 *hashes, hipStream_t s, u32 *chg) {
   if (blocks <= 0) return hipSuccess;
   const dim3 grid((unsigned)blocks);
   switch (seg) {
-    case 8: hipLaunchKernelGGL(wide_kernel<8>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes); break;
-    case 16: hipLaunchKernelGGL(wide_kernel<16>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes); break;
-    default: hipLaunchKernelGGL(wide_kernel<32>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes); break;
+    case 8: hipLaunchKernelGGL(wide_kernel<8>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes, chg); break;
+    case 16:
+      hipLaunchKernelGGL(wide_kernel<16>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes, chg);
+      break;
+    default:
+      hipLaunchKernelGGL(wide_kernel<32>, grid, dim3(256), 0, s, ents, bpre, blk2ent, k, rule, hashes, chg);
+      break;
   }
   return hipGetLastError();
 }
