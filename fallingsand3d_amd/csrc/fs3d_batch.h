@@ -40,6 +40,10 @@ __host__ __device__ inline long long world_waves_dev(int ny, int nz, int rqpw, i
 hipError_t launch_wide(int seg, const BWorld *ents, const long long *bpre, const int *blk2ent, long long blocks, int k,
                        const u64 *rule, u64 *hashes, hipStream_t s, u32 *chg = nullptr);
 hipError_t launch_expand(const long long *bpre, int n, int *blk2ent, hipStream_t s);
+// Tables of K launches in one pass (bpre: K prefixes of n+1; koff: K+1 launch
+// offsets into blk2ent; total = koff[K]).
+hipError_t launch_expand_round(const long long *bpre, int n, int K, const long long *koff, long long total,
+                               int *blk2ent, hipStream_t s);
 hipError_t launch_zero_slots(u64 *hashes, const BWorld *ents, int n, hipStream_t s);
 long long world_waves(const BWorld &w, int p);
 long long world_blocks(const BWorld &w, int p);

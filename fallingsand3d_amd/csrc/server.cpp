@@ -50,8 +50,8 @@ struct Request {
 };
 
 struct Round {
-  u8 *h_buf = nullptr;  // pinned: entries | bpre
-  u8 *d_buf = nullptr;  // device: entries | bpre
+  u8 *h_buf = nullptr;  // pinned: entries | bpre (K x (n+1)) | launch offsets (K+1)
+  u8 *d_buf = nullptr;  // device copy of the same
   size_t cap = 0;
   int *d_blk = nullptr;  // device: blk2ent for all launches
   size_t blk_cap = 0;
@@ -300,19 +300,21 @@ int launch_round(fs3d_server *s) {
   std::vector<long long> fin;
   for (int i = 0; i < n; i++)
     if (s->reqs[s->active[i]].remaining == ents[i].nt) fin.push_back(s->active[i]);
-  if (int rc = ensure_round_cap(rd, ent_bytes + pre_bytes, blks, (int)fin.size(), K + 1)) return rc;
+  const size_t koff_bytes = sizeof(long long) * (size_t)(K + 1);
+  if (int rc = ensure_round_cap(rd, ent_bytes + pre_bytes + koff_bytes, blks, (int)fin.size(), K + 1)) return rc;
   std::copy(ents.begin(), ents.end(), (BWorld *)rd.h_buf);
   std::copy(pre.begin(), pre.end(), (long long *)(rd.h_buf + ent_bytes));
-  FS3D_HIP(hipMemcpyAsync(rd.d_buf, rd.h_buf, ent_bytes + pre_bytes, hipMemcpyHostToDevice, s->s_tick));
+  long long *koff = (long long *)(rd.h_buf + ent_bytes + pre_bytes);
+  koff[0] = 0;
+  for (int k = 0; k < K; k++) koff[k + 1] = koff[k] + tot[k];
+  FS3D_HIP(hipMemcpyAsync(rd.d_buf, rd.h_buf, ent_bytes + pre_bytes + koff_bytes, hipMemcpyHostToDevice, s->s_tick));
   const BWorld *d_ents = (const BWorld *)rd.d_buf;
   const long long *d_pre = (const long long *)(rd.d_buf + ent_bytes);
-  size_t boff = 0;
-  for (int k = 0; k < K; k++) {
-    FS3D_HIP(launch_expand(d_pre + (size_t)k * (n + 1), n, rd.d_blk + boff, s->s_tick));
-    boff += (size_t)tot[k];
-  }
+  // one pass fills the workgroup -> entry tables of all K launches
+  FS3D_HIP(launch_expand_round(d_pre, n, K, (const long long *)(rd.d_buf + ent_bytes + pre_bytes), (long long)blks,
+                               rd.d_blk, s->s_tick));
   const u64 *rule = device_rule_table(s->dev);
-  boff = 0;
+  size_t boff = 0;
   FS3D_HIP(hipEventRecord(rd.ev[0], s->s_tick));
   for (int k = 0; k < K; k++) {
     FS3D_HIP(launch_wide(s->seg, d_ents, d_pre + (size_t)k * (n + 1), rd.d_blk + boff, tot[k], k, rule, s->d_hashes,

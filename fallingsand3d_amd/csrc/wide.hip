@@ -60,6 +60,28 @@ __global__ __launch_bounds__(256) void expand_kernel(const long long *__restrict
     for (long long b = bpre[i]; b < bpre[i + 1]; b++) blk2ent[b] = i;
 }
 
+// The workgroup -> entry tables of K launches at once: launch k's workgroups are
+// [koff[k], koff[k+1]) of blk2ent and its prefix is bpre[k*(n+1) .. k*(n+1)+n];
+// one thread per workgroup finds its entry by binary search (the last i with
+// bpre[i] <= b: an entry with blocks covering b has bpre[i+1] > b).
+__global__ __launch_bounds__(256) void expand_round_kernel(const long long *__restrict__ bpre, int n, int K,
+                                                           const long long *__restrict__ koff, long long total,
+                                                           int *blk2ent) {
+  for (long long gb = (long long)blockIdx.x * blockDim.x + threadIdx.x; gb < total;
+       gb += (long long)gridDim.x * blockDim.x) {
+    int k = 0;
+    while (k + 1 < K && koff[k + 1] <= gb) k++;
+    const long long b = gb - koff[k];
+    const long long *pre = bpre + (size_t)k * (n + 1);
+    int lo = 0, hi = n;  // pre[lo] <= b < pre[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid] <= b) lo = mid; else hi = mid;
+    }
+    blk2ent[gb] = lo;
+  }
+}
+
 // hashes[ents[i].slot] = 0 for i < n (before a final tick accumulates into them).
 __global__ void zero_slots_kernel(u64 *hashes, const BWorld *__restrict__ ents, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,6 +110,15 @@ hipError_t launch_expand(const long long *bpre, int n, int *blk2ent, hipStream_t
   if (n <= 0) return hipSuccess;
   const int grid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
   hipLaunchKernelGGL(expand_kernel, dim3(grid), dim3(256), 0, s, bpre, n, blk2ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_round(const long long *bpre, int n, int K, const long long *koff, long long total,
+                               int *blk2ent, hipStream_t s) {
+  if (total <= 0 || n <= 0) return hipSuccess;
+  const long long want = (total + 255) / 256;
+  const unsigned grid = (unsigned)(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(expand_round_kernel, dim3(grid), dim3(256), 0, s, bpre, n, K, koff, total, blk2ent);
   return hipGetLastError();
 }
 
