@@ -34,6 +34,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 BYTES_PER_UPDATE = 2
+# Bytes per tick of the timed region from rocprofv3 counter runs of this same
+# command (profiles/tick_kernel_boxes512.md): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB
+# per dispatch (factors from a copy of known size), averaged over its skipping
+# and hashing ticks.
+PMC_TRAFFIC_BOXES512 = 179.1e6
 METRIC = "Gvoxel-updates/s + %HBM roofline at 512^3/2048^3; batched vs plain; p99 tick ms"
 
 
@@ -333,7 +338,9 @@ def main():
                                "over a floor; ticks in one HIP graph with a fused state hash every 50 ticks",
                    "dims": [512, 512, 512], "hash_every": 50, "segment_bytes": r["seg"]},
         "roofline": {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(frac, 4), "traffic": None,
+                     "frac": round(frac, 4), "traffic": PMC_TRAFFIC_BOXES512,
+                     "traffic_source": "profiles/tick_kernel_boxes512.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "bytes per tick)",
                      "kernel": f"tick_kernel<{r['seg']},*,skip> ({sched} schedule), every tick of the timed region",
                      "ms_per_tick_device": round(r["dev_ms"], 5),
                      "alg_bytes_per_tick": int(alg_bytes / args.steps),
