@@ -129,6 +129,12 @@ SIGNATURES = {
     "fs3d_camera_globaldata": (c_int, [P(c_float), P(c_float), P(c_float)]),
     "fs3d_camera_integrate": (c_int, [P(c_float), P(c_float), c_u32]),
     "fs3d_palette": (c_int, [P(c_float)]),
+    "fs3d_rule_hash": (c_int, [P(c_u64)]),
+    "fs3d_file_write": (c_int, [ctypes.c_char_p, P(c_i32), c_u64, c_u64, vp, c_size]),
+    "fs3d_file_info": (c_int, [ctypes.c_char_p, P(c_i32), P(c_u64), P(c_u64)]),
+    "fs3d_file_read": (c_int, [ctypes.c_char_p, vp, c_size]),
+    "fs3d_world_save": (c_int, [vp, ctypes.c_char_p]),
+    "fs3d_world_load": (c_int, [vp, ctypes.c_char_p]),
 }
 
 _LIB = None

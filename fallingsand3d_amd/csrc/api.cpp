@@ -548,6 +548,49 @@ int fs3d_world_device_ptr(fs3d_world *w, const void **dptr) {
   return clear_error();
 }
 
+static int save_chunk(void *ctx, size_t off, u8 *buf, size_t n) {
+  fs3d_world *w = static_cast<fs3d_world *>(ctx);
+  FS3D_HIP(hipMemcpyAsync(buf, w->buf[w->cur] + off, n, hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return FS3D_OK;
+}
+
+static int load_chunk(void *ctx, size_t off, const u8 *buf, size_t n) {
+  fs3d_world *w = static_cast<fs3d_world *>(ctx);
+  FS3D_HIP(hipMemcpyAsync(w->buf[w->cur] + off, buf, n, hipMemcpyHostToDevice, w->stream));
+  FS3D_HIP(hipStreamSynchronize(w->stream));
+  return FS3D_OK;
+}
+
+int fs3d_world_save(fs3d_world *w, const char *path) {
+  if (int rc = check_world(w)) return rc;
+  u64 h = 0;
+  if (int rc = fs3d_world_hash(w, &h)) return rc;
+  const int32_t dims[3] = {w->nx, w->ny, w->nz};
+  if (int rc = file_write(path, dims, w->tick, h, nullptr, w->n, save_chunk, w)) return rc;
+  return clear_error();
+}
+
+int fs3d_world_load(fs3d_world *w, const char *path) {
+  if (int rc = check_world(w)) return rc;
+  int32_t dims[3];
+  u64 tick = 0, want = 0;
+  if (int rc = file_info(path, dims, &tick, &want, nullptr)) return rc;
+  if (dims[0] != w->nx || dims[1] != w->ny || dims[2] != w->nz)
+    return set_error(FS3D_ERR_INVALID, "%s holds a %dx%dx%d world, this one is %dx%dx%d", path, dims[0], dims[1],
+                     dims[2], w->nx, w->ny, w->nz);
+  FS3D_HIP(hipSetDevice(w->dev));
+  if (int rc = file_read(path, nullptr, w->n, load_chunk, w)) return rc;
+  w->tick = tick;
+  if (int rc = mark_all(w)) return rc;
+  u64 h = 0;
+  if (int rc = fs3d_world_hash(w, &h)) return rc;
+  if (h != want)
+    return set_error(FS3D_ERR_STATE, "%s: grid hash %016llx does not match the stored %016llx", path,
+                     (unsigned long long)h, (unsigned long long)want);
+  return clear_error();
+}
+
 int fs3d_world_fill_rain(fs3d_world *w, uint64_t seed, uint64_t thr) {
   if (int rc = check_world(w)) return rc;
   FS3D_HIP(hipSetDevice(w->dev));

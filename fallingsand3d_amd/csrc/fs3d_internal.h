@@ -52,6 +52,16 @@ constexpr u64 kRuleChanged = 0x80ull;
 void build_rule_entries(u64 *entries);
 u16 rule_state_packed(u16 s, int v);
 
+// ---- world files ------------------------------------------------------------------
+u64 rule_hash();
+// fill (when grid is NULL) provides the payload in chunks; sink (when grid is
+// NULL) receives it.  Both return an fs3d status.
+int file_write(const char *path, const int32_t dims[3], u64 tick, u64 state_hash, const u8 *grid, size_t nbytes,
+               int (*fill)(void *ctx, size_t off, u8 *buf, size_t n), void *ctx);
+int file_info(const char *path, int32_t dims[3], u64 *tick, u64 *state_hash, u64 *rule);
+int file_read(const char *path, u8 *grid, size_t nbytes, int (*sink)(void *ctx, size_t off, const u8 *buf, size_t n),
+              void *ctx);
+
 // ---- device context -------------------------------------------------------------
 int ensure_device(int dev);            // sets the device current, uploads the rule table
 const u64 *device_rule_table(int dev);  // valid after ensure_device

@@ -74,6 +74,37 @@ def serve_specs(n, seed, lo=32, hi=256, tick_lo=10, tick_hi=500, lam=2000.0):
     return dims.reshape(n, 3), thr, seeds, ticks, arr
 
 
+def rule_hash():
+    """Hash of the tick's block rule (stored in world files)."""
+    h = c_u64(0)
+    check(lib().fs3d_rule_hash(ctypes.byref(h)))
+    return h.value
+
+
+def file_write(path, grid, tick=0, state_hash=0):
+    """Writes a .fs3d world file from a host grid g[z][y][x] (no GPU needed)."""
+    g = np.ascontiguousarray(grid, dtype=np.uint8)
+    nz, ny, nx = g.shape
+    d = (c_i32 * 3)(nx, ny, nz)
+    check(lib().fs3d_file_write(str(path).encode(), d, tick, state_hash, g.ctypes.data, g.size))
+
+
+def file_info(path):
+    """(dims (nx, ny, nz), tick, state hash) from a .fs3d header."""
+    d = (c_i32 * 3)()
+    t, h = c_u64(0), c_u64(0)
+    check(lib().fs3d_file_info(str(path).encode(), d, ctypes.byref(t), ctypes.byref(h)))
+    return (d[0], d[1], d[2]), t.value, h.value
+
+
+def file_read(path):
+    """The grid g[z][y][x] of a .fs3d file (refused if written under another rule)."""
+    (nx, ny, nz), _, _ = file_info(path)
+    g = np.empty((nz, ny, nx), np.uint8)
+    check(lib().fs3d_file_read(str(path).encode(), g.ctypes.data, g.size))
+    return g
+
+
 class World:
     """A uint8 voxel grid of dims (nx, ny, nz) on the current GPU."""
 
@@ -210,6 +241,14 @@ class World:
     @skip.setter
     def skip(self, mode):
         check(lib().fs3d_world_set_skip(self.handle, int(mode)))
+
+    def save(self, path):
+        """Checkpoints the grid and tick counter to a .fs3d file."""
+        check(lib().fs3d_world_save(self.handle, str(path).encode()))
+
+    def load(self, path):
+        """Resumes from a .fs3d file of the same dims (grid, tick; hashes checked)."""
+        check(lib().fs3d_world_load(self.handle, str(path).encode()))
 
     def sweep_stats(self, enable=True):
         """Counters since the previous call: voxels read and written, voxels only

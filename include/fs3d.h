@@ -291,6 +291,38 @@ int fs3d_server_replay(fs3d_server *s, int32_t n, const int32_t *dims, const uin
                        const int32_t *ticks, const double *arrival, int64_t *ids);
 int fs3d_server_get_stats(const fs3d_server *s, fs3d_server_stats *out);
 
+/* ---- world files (checkpoint / resume) ----------------------------------------
+ * No reference counterpart (the reference keeps no simulation state; its assets
+ * are meshes and textures).  A .fs3d file is a 64-byte little-endian header and
+ * the raw grid:
+ *    0  char[8]  "FS3DGRID"
+ *    8  u32      format version (1)
+ *   12  u32      header bytes (64)
+ *   16  i32[3]   nx, ny, nz
+ *   28  u32      flags (0)
+ *   32  u64      tick counter
+ *   40  u64      rule hash (fs3d_rule_hash of the library that wrote it)
+ *   48  u64      state hash of the grid (as fs3d_world_hash)
+ *   56  u64      payload bytes (nx*ny*nz)
+ *   64  u8[]     v[z][y][x]
+ * Loading refuses a file whose rule hash differs from this library's (the same
+ * bytes would evolve differently) and one whose payload does not match its state
+ * hash. */
+#define FS3D_FILE_VERSION 1
+/* Hash of the tick's block rule: SplitMix64-mixed sum over the packed table. */
+int fs3d_rule_hash(uint64_t *hash);
+/* Host-only file access (no device needed). */
+int fs3d_file_write(const char *path, const int32_t dims[3], uint64_t tick, uint64_t state_hash, const uint8_t *grid,
+                    size_t nbytes);
+/* Reads the header; dims, tick and state_hash may be NULL. */
+int fs3d_file_info(const char *path, int32_t dims[3], uint64_t *tick, uint64_t *state_hash);
+/* Reads the payload into grid (nbytes must equal nx*ny*nz). */
+int fs3d_file_read(const char *path, uint8_t *grid, size_t nbytes);
+/* Saves the world's current grid and tick (synchronises). */
+int fs3d_world_save(fs3d_world *w, const char *path);
+/* Loads a file into a world of the same dims: grid, tick counter, checked hashes. */
+int fs3d_world_load(fs3d_world *w, const char *path);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Average device time (ms) of the tick kernels launched by the last
  * fs3d_world_run on this world, measured with HIP events on the world's stream

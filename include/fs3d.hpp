@@ -99,6 +99,10 @@ class World {
   }
   void sync() { check(fs3d_world_sync(w_)); }
 
+  void save(const std::string &path) { check(fs3d_world_save(w_, path.c_str())); }
+  void load(const std::string &path) { check(fs3d_world_load(w_, path.c_str())); }
+  void set_skip(int32_t mode) { check(fs3d_world_set_skip(w_, mode)); }
+
   void fill_rain(uint64_t seed, uint64_t thr) { check(fs3d_world_fill_rain(w_, seed, thr)); }
   int64_t fill_boxes(uint64_t seed, const fs3d_box_params &p) {
     int64_t n = 0;
