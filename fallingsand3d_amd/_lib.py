@@ -123,6 +123,8 @@ SIGNATURES = {
     "fs3d_server_replay": (c_int, [vp, c_i32, P(c_i32), P(c_u64), P(c_u64), P(c_i32), P(c_double), P(c_i64)]),
     "fs3d_server_get_stats": (c_int, [vp, P(ServerStats)]),
     "fs3d_render_sdf": (c_int, [P(c_float), c_float, c_i32, c_i32, P(c_float)]),
+    "fs3d_box_vertex": (c_int, [P(c_float), P(c_float), P(c_float)]),
+    "fs3d_render_box_march": (c_int, [P(c_float), P(c_float), c_i32, P(c_float)]),
     "fs3d_world_render": (c_int, [vp, P(Camera), c_i32, c_i32, P(c_float), P(c_u64), P(c_u8)]),
     "fs3d_world_last_render_ms": (c_int, [vp, P(c_double)]),
     "fs3d_orbit_camera": (c_int, [P(c_i32), c_double, c_double, c_double, c_float, P(Camera)]),

@@ -22,5 +22,6 @@ size_t occ_words(int nx, int ny, int nz);
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
                       u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s);
 void palette_host(float *rgba);
+hipError_t launch_box_march(const float *d_origin, const float *d_local, int n, float *d_rgba, hipStream_t s);
 
 }  // namespace fs3d

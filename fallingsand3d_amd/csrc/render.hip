@@ -92,6 +92,42 @@ __global__ __launch_bounds__(256) void sdf_kernel(float ox, float oy, float oz, 
   out[3] = 1.0f;
 }
 
+// shaders/raymarch_sphere.frag:9-53 -- the scene's active marcher: per fragment,
+// dir = normalize(local - origin), at most 64 box-SDF steps (half-size 0.5), hit
+// below 0.001 -> (max(0.05, density), 0, 0) with density 1 - 0.05 per step taken,
+// stop once t > 1000 (tested after the hit test), miss = black, alpha 1.
+__device__ __forceinline__ float box_sdf(float x, float y, float z) {
+  const float qx = fabsf(x) - 0.5f, qy = fabsf(y) - 0.5f, qz = fabsf(z) - 0.5f;
+  const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+  const float len = sqrtf((mx * mx + my * my) + mz * mz);
+  return len + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+}
+
+__global__ __launch_bounds__(256) void box_march_kernel(const float *__restrict__ origin,
+                                                        const float *__restrict__ local, int n, float *rgba) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const float o0 = origin[3 * k], o1 = origin[3 * k + 1], o2 = origin[3 * k + 2];
+  const F3 d = nrm3(local[3 * k] - o0, local[3 * k + 1] - o1, local[3 * k + 2] - o2);
+  float r = 0.0f, t = 0.0f, density = 1.0f;
+  for (int i = 0; i < 64; i++) {
+    const float dist = box_sdf(o0 + d.x * t, o1 + d.y * t, o2 + d.z * t);
+    if (dist < 0.001f) {
+      r = fmaxf(0.05f, density);
+      break;
+    } else if (t > 1000.0f) {
+      break;
+    }
+    t = t + dist;
+    density = density - 0.05f;
+  }
+  float *out = rgba + (size_t)k * 4;
+  out[0] = r;
+  out[1] = 0.0f;
+  out[2] = 0.0f;
+  out[3] = 1.0f;
+}
+
 __device__ __forceinline__ int palette_index(u32 m) {
   return m == FS3D_STATIC ? 127 : m == FS3D_SAND ? 162 : m == FS3D_FLUID ? 137 : (int)m;
 }
@@ -257,6 +293,12 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
 hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float *d_rgba, hipStream_t s) {
   const dim3 grid((W + 15) / 16, (H + 15) / 16);
   hipLaunchKernelGGL(sdf_kernel, grid, dim3(256), 0, s, cam_pos[0], cam_pos[1], cam_pos[2], aspect, W, H, d_rgba);
+  return hipGetLastError();
+}
+
+hipError_t launch_box_march(const float *d_origin, const float *d_local, int n, float *d_rgba, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(box_march_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_origin, d_local, n, d_rgba);
   return hipGetLastError();
 }
 

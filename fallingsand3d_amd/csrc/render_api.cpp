@@ -215,6 +215,33 @@ int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, 
   return clear_error();
 }
 
+int fs3d_box_vertex(const float cam_pos[3], const float inv_model[16], float origin[3]) {
+  if (!cam_pos || !inv_model || !origin) return set_error(FS3D_ERR_INVALID, "box_vertex: NULL argument");
+  const float v[4] = {cam_pos[0] * -1.0f, cam_pos[1] * -1.0f, cam_pos[2] * -1.0f, 1.0f};
+  for (int i = 0; i < 3; i++)
+    origin[i] = ((inv_model[0 * 4 + i] * v[0] + inv_model[1 * 4 + i] * v[1]) + inv_model[2 * 4 + i] * v[2]) +
+                inv_model[3 * 4 + i] * v[3];
+  return clear_error();
+}
+
+int fs3d_render_box_march(const float *origin, const float *local, int32_t n, float *rgba) {
+  if (!origin || !local || !rgba || n < 0) return set_error(FS3D_ERR_INVALID, "box_march: bad arguments");
+  if (n == 0) return clear_error();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (int rc = ensure_device(dev)) return rc;
+  const size_t in_bytes = (size_t)n * 3 * sizeof(float), out_bytes = (size_t)n * 4 * sizeof(float);
+  float *d = nullptr;
+  FS3D_HIP(hipMalloc(&d, 2 * in_bytes + out_bytes));
+  hipError_t e = hipMemcpy(d, origin, in_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + 3 * (size_t)n, local, in_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_box_march(d, d + 3 * (size_t)n, n, d + 6 * (size_t)n, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(rgba, d + 6 * (size_t)n, out_bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "box_march failed: %s", hipGetErrorString(e));
+  return clear_error();
+}
+
 int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t H, float *rgb, uint64_t *hit,
                       uint8_t *face) {
   if (!w) return set_error(FS3D_ERR_INVALID, "world handle is NULL");

@@ -28,6 +28,25 @@ def render_sdf(cam_pos=(0.0, 0.0, -5.0), aspect=1700.0 / 900.0, W=850, H=450):
     return out
 
 
+def box_vertex(cam_pos, inv_model):
+    """Ray origin of shaders/raymarch_sphere.vert: invModel * (-camPos, 1) (inv_model
+    column-major 4x4)."""
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    m = np.ascontiguousarray(inv_model, np.float32).reshape(16)
+    out = np.zeros(3, np.float32)
+    check(lib().fs3d_box_vertex(ptr(cp, c_float), ptr(m, c_float), ptr(out, c_float)))
+    return out
+
+
+def box_march(origins, locals_):
+    """shaders/raymarch_sphere.frag on the GPU for n fragments: (n, 4) RGBA."""
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    l = np.ascontiguousarray(locals_, np.float32).reshape(-1, 3)
+    out = np.zeros((len(o), 4), np.float32)
+    check(lib().fs3d_render_box_march(ptr(o, c_float), ptr(l, c_float), len(o), ptr(out, c_float)))
+    return out
+
+
 def orbit_camera(dims, angle, dist_scale=1.6, elev=0.4, aspect=16.0 / 9.0):
     d = (c_i32 * 3)(*[int(v) for v in dims])
     cam = Camera()

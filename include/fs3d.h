@@ -214,6 +214,14 @@ typedef struct fs3d_camera {
 /* HIP port of shaders/fs_raymarch.frag (sphere tracer; origin = cam_pos as in
  * shaders/fs_raymarch.vert:33): rgba[H][W][4] float, written to host memory. */
 int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, float *rgba);
+/* The scene's active marcher, shaders/raymarch_sphere.{vert,frag}, at its stage
+ * boundaries (the rasteriser between them stays the caller's).  Vertex stage:
+ * the fragments' ray origin, invModel * vec4(-camPos, 1) (inv_model column-major
+ * as in src/engine/rendering/materials.h:30-33; host only).  Fragment stage: n
+ * fragments given their origin[3] and interpolated object-space position
+ * local[3]; rgba[4] each. */
+int fs3d_box_vertex(const float cam_pos[3], const float inv_model[16], float origin[3]);
+int fs3d_render_box_march(const float *origin, const float *local, int32_t n, float *rgba);
 /* Voxel ray caster over the world's current grid (the grid -> renderer hand-off
  * the reference's set-2 bindings would carry, src/engine/rendering/materials.cpp:26-59):
  * rgb[H][W][3] float, hit[H][W] = linear voxel index of the first non-EMPTY cell

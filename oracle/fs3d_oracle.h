@@ -115,6 +115,10 @@ typedef struct {
 } fso_camera;
 void fso_pixel_dir(int px, int py, int W, int H, float aspect, float d[3]);
 void fso_raymarch_sdf(const float cam_pos[3], float aspect, int W, int H, float *rgba);
+/* shaders/raymarch_sphere.{vert,frag}: object-space ray origin of the vertex
+ * stage, and the fragment stage over n (origin, local position) pairs. */
+void fso_box_vertex(const float cam_pos[3], const float inv_model[16], float origin[3]);
+void fso_box_march(const float *origin, const float *local, int n, float *rgba);
 void fso_render_dda(const uint8_t *g, int nx, int ny, int nz, const fso_camera *cam, const float *palette, int W,
                     int H, float *rgb, uint64_t *hit, uint8_t *face);
 void fso_orbit_camera(int nx, int ny, int nz, double angle, double dist_scale, double elev, float aspect,

@@ -252,3 +252,54 @@ void fso_orbit_camera(int nx, int ny, int nz, double angle, double dist_scale, d
   out->light[2] = (float)(c[2] - 3.0 * m);
   out->aspect = aspect;
 }
+
+/*
+ * fso_box_vertex / fso_box_march restate the scene's active ray marcher
+ * (shaders/raymarch_sphere.vert:30-45 and shaders/raymarch_sphere.frag:9-53).
+ * Vertex stage: the fragment's ray origin is the camera in object space,
+ * invModel * vec4(-camPos, 1) (column-major; each output component summed
+ * ((m0*v0 + m1*v1) + m2*v2) + m3*v3).  Fragment stage: dir = normalize(local -
+ * origin); march t from 0 for at most 64 steps through the box SDF of half-size
+ * 0.5, d = length(max(q,0)) + min(max(q.x, max(q.y, q.z)), 0) with q = |p| - 0.5;
+ * a hit (d < 0.001) gives (max(0.05, density), 0, 0) where density starts at 1
+ * and drops by 0.05 per step taken; t > 1000 (tested after the hit test) or 64
+ * steps give black; alpha 1.
+ */
+void fso_box_vertex(const float cam_pos[3], const float inv_model[16], float origin[3]) {
+  const float v[4] = {cam_pos[0] * -1.0f, cam_pos[1] * -1.0f, cam_pos[2] * -1.0f, 1.0f};
+  for (int i = 0; i < 3; i++)
+    origin[i] = ((inv_model[0 * 4 + i] * v[0] + inv_model[1 * 4 + i] * v[1]) + inv_model[2 * 4 + i] * v[2]) +
+                inv_model[3 * 4 + i] * v[3];
+}
+
+static float box_sdf(float x, float y, float z) {
+  const float qx = fabsf(x) - 0.5f, qy = fabsf(y) - 0.5f, qz = fabsf(z) - 0.5f;
+  const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+  const float len = sqrtf((mx * mx + my * my) + mz * mz);
+  return len + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+}
+
+void fso_box_march(const float *origin, const float *local, int n, float *rgba) {
+  for (int k = 0; k < n; k++) {
+    const float *o = origin + 3 * k, *l = local + 3 * k;
+    float d[3];
+    nrm3(l[0] - o[0], l[1] - o[1], l[2] - o[2], d);
+    float r = 0.0f, t = 0.0f, density = 1.0f;
+    for (int i = 0; i < 64; i++) {
+      const float px = o[0] + d[0] * t, py = o[1] + d[1] * t, pz = o[2] + d[2] * t;
+      const float dist = box_sdf(px, py, pz);
+      if (dist < 0.001f) {
+        r = fmaxf(0.05f, density);
+        break;
+      } else if (t > 1000.0f) {
+        break;
+      }
+      t = t + dist;
+      density = density - 0.05f;
+    }
+    rgba[4 * k + 0] = r;
+    rgba[4 * k + 1] = 0.0f;
+    rgba[4 * k + 2] = 0.0f;
+    rgba[4 * k + 3] = 1.0f;
+  }
+}

@@ -83,6 +83,8 @@ def lib():
                                        i32p, f64p]),
             "fso_pixel_dir": (None, [ctypes.c_int] * 4 + [ctypes.c_float, f32p]),
             "fso_raymarch_sdf": (None, [f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int, f32p]),
+            "fso_box_vertex": (None, [f32p, f32p, f32p]),
+            "fso_box_march": (None, [f32p, f32p, ctypes.c_int, f32p]),
             "fso_render_dda": (None, [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Camera), f32p,
                                       ctypes.c_int, ctypes.c_int, f32p, u64p, u8p]),
             "fso_orbit_camera": (None, [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_float,
@@ -253,6 +255,24 @@ def raymarch_sdf(cam_pos, aspect, W, H):
     cp = np.ascontiguousarray(cam_pos, np.float32)
     out = np.zeros((H, W, 4), np.float32)
     lib().fso_raymarch_sdf(_p(cp, f32p), aspect, W, H, _p(out, f32p))
+    return out
+
+
+def box_vertex(cam_pos, inv_model):
+    """Object-space ray origin of shaders/raymarch_sphere.vert (inv_model column-major 4x4)."""
+    cp = np.ascontiguousarray(cam_pos, np.float32)
+    m = np.ascontiguousarray(inv_model, np.float32).reshape(16)
+    out = np.zeros(3, np.float32)
+    lib().fso_box_vertex(_p(cp, f32p), _p(m, f32p), _p(out, f32p))
+    return out
+
+
+def box_march(origins, locals_):
+    """RGBA (n, 4) of shaders/raymarch_sphere.frag for n (origin, local position) pairs."""
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    l = np.ascontiguousarray(locals_, np.float32).reshape(-1, 3)
+    out = np.zeros((len(o), 4), np.float32)
+    lib().fso_box_march(_p(o, f32p), _p(l, f32p), len(o), _p(out, f32p))
     return out
 
 

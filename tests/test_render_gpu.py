@@ -90,3 +90,18 @@ def test_render_hand_off_after_ticks(gpu, O, pal):
         cam = R.orbit_camera(w.dims, 1.1, 1.5, 0.5, 16 / 9)
         _compare(R, O, w, cam, pal, 480, 270, "ticked")
         assert R.last_render_ms(w) > 0
+
+
+def test_box_march_port_matches_transcription(gpu, O):
+    from fallingsand3d_amd import render as R
+    announce("raymarch_sphere fragment stage: 200k fragments vs the transcription")
+    rng = np.random.default_rng(13)
+    n = 200000
+    origins = (rng.normal(size=(n, 3)) * rng.uniform(0.2, 6, size=(n, 1))).astype(np.float32)
+    locals_ = rng.uniform(-0.5, 0.5, size=(n, 3)).astype(np.float32)
+    face = rng.integers(0, 3, size=n)
+    locals_[np.arange(n // 2), face[: n // 2]] = np.sign(origins[np.arange(n // 2), face[: n // 2]]) * 0.5
+    got = R.box_march(origins, locals_)
+    want = O.box_march(origins, locals_)
+    assert np.abs(got - want).max() <= RGB_TOL
+    assert (got[:, 0] > 0).mean() > 0.3
