@@ -89,6 +89,8 @@ SIGNATURES = {
     "fs3d_world_get_segment": (c_int, [vp, P(c_i32)]),
     "fs3d_world_set_skip": (c_int, [vp, c_i32]),
     "fs3d_world_get_skip": (c_int, [vp, P(c_i32)]),
+    "fs3d_world_set_fuse": (c_int, [vp, c_i32]),
+    "fs3d_world_get_fuse": (c_int, [vp, P(c_i32)]),
     "fs3d_world_sweep_stats": (c_int, [vp, c_i32, P(c_u64)]),
     "fs3d_world_fill_rain": (c_int, [vp, c_u64, c_u64]),
     "fs3d_world_fill_boxes": (c_int, [vp, c_u64, P(BoxParams), P(c_i64)]),

@@ -173,25 +173,68 @@ bool skip_listed(const fs3d_world *w) {
   return w->skip == FS3D_SKIP_LISTED || (w->skip == FS3D_SKIP_AUTO && w->shape.waves > (1ll << 17));
 }
 
-int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev) {
+bool fuse_ok(const fs3d_world *w) { return !w->skip && w->fuse && tick2_supported(w->shape2, w->nx, nullptr); }
+
+// Grids up to this many voxels keep both buffers within about twice the 256 MB
+// MALL, where the single-tick kernel is not HBM-bound and fusion's redundant
+// halo work costs more than the traffic it saves (512^3: 74.5 vs 46.5 us per
+// tick; 1024^3: 369 vs 402 us; 2048^3: 2.57 vs 3.26 ms).
+constexpr u64 kFuseMinVoxels = 1ull << 28;
+
+// The fused pass's launch shape: the smallest segment that puts a whole row in
+// one wave (it shares no state with the single-tick kernel's shape), and the
+// default for pairing ticks.
+void plan_fused(fs3d_world *w) {
+  for (int seg : {8, 16, 32, 64})
+    if ((w->nx + seg - 1) / seg <= 64) {
+      (void)plan_tick(w->nx, w->ny, w->nz, seg, &w->shape2);
+      break;
+    }
+  // 64-byte segments (nx > 2048) hold 211 VGPRs per lane: two waves per SIMD,
+  // too few to hide the rule-table gathers, so they are opt-in
+  w->fuse = (u64)w->nx * (u64)w->ny * (u64)w->nz > kFuseMinVoxels && w->shape2.seg <= 32;
+}
+
+// z row-quads per workgroup of the fused two-tick pass
+constexpr int kTick2ZLen = 32;
+
+// Enqueues nticks ticks starting from buffer w->cur; *final receives the index of
+// the buffer holding the result (a fused pair of dense ticks moves the state to
+// the other buffer once, a single tick once).  With launch = false it only
+// computes *final.
+int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev, int *final, bool launch = true) {
   const u64 *rule = device_rule_table(w->dev);
   u64 *const *masks = w->skip ? w->d_masks : nullptr;
-  const bool listed = skip_listed(w);
-  for (int k = 0; k < nticks; k++) {
-    const bool hashed = hash_every > 0 && (k + 1) % hash_every == 0;
-    const int src = (w->cur + k) & 1;
-    if (masks && listed && !hashed) {
-      // settled row quads are listed out; hash ticks read every segment anyway
-      FS3D_HIP(launch_tick_listed(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
-                                  w->tick + (u64)k, tick_dev, (u32)k, rule, w->stream, masks,
-                                  w->d_stats, w->d_list, w->d_ctl, w->mask_words, w->list_waves));
+  const bool listed = skip_listed(w), fuse = fuse_ok(w);
+  auto hashed_after = [&](int k) { return hash_every > 0 && (k + 1) % hash_every == 0; };
+  int src = w->cur;
+  for (int k = 0; k < nticks;) {
+    if (fuse && k + 1 < nticks && !hashed_after(k) && !hashed_after(k + 1)) {
+      if (launch)
+        FS3D_HIP(launch_tick2(w->shape2, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz, w->tick + (u64)k,
+                              tick_dev, (u32)k, rule, kTick2ZLen, w->stream));
+      src ^= 1;
+      k += 2;
       continue;
     }
-    FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
-                         w->tick + (u64)k, tick_dev, (u32)k, rule,
-                         hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
-    if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
+    const bool hashed = hashed_after(k);
+    if (launch) {
+      if (masks && listed && !hashed) {
+        // settled row quads are listed out; hash ticks read every segment anyway
+        FS3D_HIP(launch_tick_listed(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                                    w->tick + (u64)k, tick_dev, (u32)k, rule, w->stream, masks,
+                                    w->d_stats, w->d_list, w->d_ctl, w->mask_words, w->list_waves));
+      } else {
+        FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                             w->tick + (u64)k, tick_dev, (u32)k, rule,
+                             hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
+        if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
+      }
+    }
+    src ^= 1;
+    k++;
   }
+  if (final) *final = src;
   return FS3D_OK;
 }
 
@@ -276,6 +319,7 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
     release(w);
     return set_error(FS3D_ERR_HIP, "world init failed: %s", hipGetErrorString(e));
   }
+  plan_fused(w);
   rc = ensure_small(w, 64);
   if (!rc) rc = size_list(w);
   if (rc) {
@@ -392,6 +436,24 @@ int fs3d_world_set_skip(fs3d_world *w, int32_t mode) {
   return clear_error();
 }
 
+int fs3d_world_set_fuse(fs3d_world *w, int32_t on) {
+  if (int rc = check_world(w)) return rc;
+  FS3D_HIP(hipSetDevice(w->dev));
+  if ((on != 0) != w->fuse) {
+    FS3D_HIP(hipStreamSynchronize(w->stream));
+    drop_graphs(w);
+    w->fuse = on != 0;
+  }
+  return clear_error();
+}
+
+int fs3d_world_get_fuse(const fs3d_world *w, int32_t *on) {
+  if (int rc = check_world(w)) return rc;
+  if (!on) return set_error(FS3D_ERR_INVALID, "on is NULL");
+  *on = fuse_ok(w) ? 1 : 0;
+  return clear_error();
+}
+
 int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode) {
   if (int rc = check_world(w)) return rc;
   if (!mode) return set_error(FS3D_ERR_INVALID, "mode is NULL");
@@ -440,8 +502,9 @@ int fs3d_world_step(fs3d_world *w, int32_t nticks) {
   if (int rc = check_world(w)) return rc;
   if (nticks < 0) return set_error(FS3D_ERR_INVALID, "nticks < 0");
   FS3D_HIP(hipSetDevice(w->dev));
-  if (int rc = enqueue_ticks(w, nticks, 0, nullptr)) return rc;
-  w->cur = (w->cur + nticks) & 1;
+  int fin = w->cur;
+  if (int rc = enqueue_ticks(w, nticks, 0, nullptr, &fin)) return rc;
+  w->cur = fin;
   w->tick += (u64)nticks;
   return clear_error();
 }
@@ -450,10 +513,11 @@ int fs3d_world_tick(fs3d_world *w, uint64_t *hash) {
   if (int rc = check_world(w)) return rc;
   if (!hash) return set_error(FS3D_ERR_INVALID, "hash is NULL");
   FS3D_HIP(hipSetDevice(w->dev));
-  if (int rc = enqueue_ticks(w, 1, 1, nullptr)) return rc;
+  int fin = w->cur;
+  if (int rc = enqueue_ticks(w, 1, 1, nullptr, &fin)) return rc;
   FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, sizeof(u64), hipMemcpyDeviceToHost, w->stream));
   FS3D_HIP(hipStreamSynchronize(w->stream));
-  w->cur ^= 1;
+  w->cur = fin;
   w->tick += 1;
   *hash = w->h_pinned[0];
   return clear_error();
@@ -484,7 +548,7 @@ int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *
   if (it == w->graphs.end()) {
     hipGraph_t graph = nullptr;
     FS3D_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_ticks(w, nticks, hash_every, w->d_small);
+    int rc = enqueue_ticks(w, nticks, hash_every, w->d_small, nullptr);
     hipError_t e = hipStreamEndCapture(w->stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -508,7 +572,9 @@ int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *
   FS3D_HIP(hipEventElapsedTime(&ms, w->ev0, w->ev1));
   w->last_run_ms = (double)ms / nticks;
   for (int i = 0; i < nh; i++) hashes[i] = w->h_pinned[i];
-  w->cur = (w->cur + nticks) & 1;
+  int fin = w->cur;
+  if (int rc = enqueue_ticks(w, nticks, hash_every, nullptr, &fin, false)) return rc;
+  w->cur = fin;
   w->tick += (u64)nticks;
   return clear_error();
 }

@@ -106,6 +106,11 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
 // persistent kernel for a segment size.
 int list_blocks_per_cu(int seg);
 hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s);
+// Two dense ticks (t and t + 1) in one pass: src -> dst holds the state after
+// both.  Supported for one row quad per wave and LDS tiles within 64 KiB.
+bool tick2_supported(const TickShape &sh, int nx, size_t *lds_bytes);
+hipError_t launch_tick2(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
+                        const u64 *tick_dev, u32 k_off, const u64 *rule, int zlen, hipStream_t s);
 hipError_t launch_hash(const u8 *g, size_t n, u64 *hash_parts, hipStream_t s);
 hipError_t launch_hash_finalize(u64 *hash_parts, u64 *out, hipStream_t s);
 hipError_t launch_set_u64(u64 *p, u64 v, hipStream_t s);
@@ -143,6 +148,7 @@ struct fs3d_world {
   fs3d::u64 *h_pinned = nullptr; // pinned staging for small read-backs
   int h_pinned_cap = 0;
   fs3d::TickShape shape{};
+  fs3d::TickShape shape2{};  // the fused two-tick pass: smallest segment with whole rows per wave
   int seg_req = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_run_ms = 0.0;
@@ -156,6 +162,7 @@ struct fs3d_world {
   // settled-segment skipping: mode (FS3D_SKIP_*) and per-parity change masks
   // (one u64 per row quad)
   int skip = FS3D_SKIP_AUTO;
+  bool fuse = false;  // dense ticks: pairs of ticks in one pass where supported
   fs3d::u64 *d_masks[2] = {nullptr, nullptr};
   size_t mask_words = 0;
   fs3d::u64 *d_stats = nullptr;  // sweep counters, allocated when enabled

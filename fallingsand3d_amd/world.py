@@ -250,6 +250,17 @@ class World:
         """Resumes from a .fs3d file of the same dims (grid, tick; hashes checked)."""
         check(lib().fs3d_world_load(self.handle, str(path).encode()))
 
+    @property
+    def fuse(self):
+        """Whether dense ticks run as fused pairs (set it to allow or forbid that)."""
+        s = c_i32(0)
+        check(lib().fs3d_world_get_fuse(self.handle, ctypes.byref(s)))
+        return bool(s.value)
+
+    @fuse.setter
+    def fuse(self, on):
+        check(lib().fs3d_world_set_fuse(self.handle, 1 if on else 0))
+
     def sweep_stats(self, enable=True):
         """Counters since the previous call: voxels read and written, voxels only
         read (settled segments on hash ticks), and settled segments that changed

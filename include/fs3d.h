@@ -121,6 +121,15 @@ int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg_bytes);
 enum { FS3D_SKIP_OFF = 0, FS3D_SKIP_AUTO = 1, FS3D_SKIP_LISTED = 2, FS3D_SKIP_WAVES = 3 };
 int fs3d_world_set_skip(fs3d_world *w, int32_t mode);
 int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode);
+/* Two-tick fusion of the dense tick (used when skipping is off): a pair of
+ * ticks, neither hashed, runs as one pass that keeps the first tick's output in
+ * LDS and writes only the second's, so the grid crosses HBM once per two ticks.
+ * On by default for grids of more than 2^28 voxels with nx <= 2048 (smaller
+ * grids stay in the last-level cache, where single ticks are faster).  Needs
+ * 256 < nx <= 3584 (whole rows per wave, LDS tiles within 64 KiB); get_fuse
+ * reports whether fusion is in effect. */
+int fs3d_world_set_fuse(fs3d_world *w, int32_t on);
+int fs3d_world_get_fuse(const fs3d_world *w, int32_t *on);
 /* Sweep counters of the skipping tick: enable != 0 starts (or keeps) counting,
  * 0 stops.  Returns and clears what was counted so far: out[0] = voxels read and
  * written, out[1] = voxels only read (settled segments on hash ticks), out[2] =
