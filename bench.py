@@ -11,9 +11,10 @@ Default run (one JSON line on stdout):
            the same ticks with sweep counters) / device time of the timed graph
            (HIP events on the world's stream), against 8.0 TB/s; "dense" holds the
            dense tick kernel on the same world (2 bytes per voxel-update).
-  extra    pillars2048: the 2048^3 pillar scene, skipping and dense (the dense
-           kernel is the HBM-bound figure: the 512^3 pair of buffers nearly fits
-           the 256 MB Infinity Cache); pillars2048_render: that scene with a
+  extra    pillars2048: the 2048^3 pillar scene, skipping, dense and dense fused
+           in pairs (the dense single-tick kernel is the HBM-bound figure: the
+           512^3 pair of buffers nearly fits the 256 MB Infinity Cache);
+           pillars2048_render: that scene with a
            1920x1080 voxel render every 10 ticks; copy: a plain device copy of the
            same bytes, the practical bandwidth ceiling; batched: 4096 worlds against
            the one-launch-per-world loop; serving: the continuous-batching server
@@ -92,31 +93,35 @@ def bench_boxes512(F, steps, warmup, kernel_ticks=200):
 
 def bench_pillars2048(F, ticks):
     """The 2048^3 pillar scene: `ticks` plain ticks in one graph with the default
-    (skipping) tick and with the dense kernel (the HBM roofline figure)."""
+    (skipping) tick, with the dense single-tick kernel (the HBM roofline figure)
+    and with dense ticks fused in pairs (two ticks per pass over the grid)."""
     n = 2048
     out = {}
-    for dense in (False, True):
-        with F.World((n, n, n), flags=F.DENSE if dense else 0) as w:
+    for kind in ("skip", "dense", "fused"):
+        with F.World((n, n, n), flags=0 if kind == "skip" else F.DENSE) as w:
+            if kind != "skip":
+                w.fuse = kind == "fused"
+                if w.fuse != (kind == "fused"):
+                    raise SystemExit("pillars2048: fusion not available at 2048^3")
             t0 = time.perf_counter()
             w.fill_pillars(2)
             log(f"[bench] pillars2048: scene generated in {time.perf_counter() - t0:.2f}s")
             w.run(4, 0)
             w.fill_pillars(2)
             w.tick_count = 0
-            if not dense:
+            if kind == "skip":
                 w.sweep_stats(True)
             t0 = time.perf_counter()
             w.run(ticks, 0)
             wall = time.perf_counter() - t0
             kern_ms = w.last_run_ms_per_tick()
-            st = w.sweep_stats(False) if not dense else None
+            st = w.sweep_stats(False) if kind == "skip" else None
             h = w.hash()
-            log(f"[bench] pillars2048 {'dense' if dense else 'skip'}: {ticks} ticks in {wall * 1e3:.1f} ms "
-                f"({kern_ms:.3f} ms/tick)")
-            out["dense" if dense else "skip"] = dict(wall=wall, kern_ms=kern_ms, hash=int(h),
-                                                     swept=st["swept"] if st else None, seg=w.segment)
-    if out["dense"]["hash"] != out["skip"]["hash"]:
-        raise SystemExit("pillars2048: skipping and dense ticks disagree")
+            log(f"[bench] pillars2048 {kind}: {ticks} ticks in {wall * 1e3:.1f} ms ({kern_ms:.3f} ms/tick)")
+            out[kind] = dict(wall=wall, kern_ms=kern_ms, hash=int(h), swept=st["swept"] if st else None,
+                             seg=w.segment)
+    if not out["dense"]["hash"] == out["skip"]["hash"] == out["fused"]["hash"]:
+        raise SystemExit("pillars2048: skipping, dense and fused ticks disagree")
     return dict(vox=n ** 3, ticks=ticks, **out)
 
 
@@ -291,7 +296,8 @@ def main():
     if args.only == "pillars2048":
         p = bench_pillars2048(F, args.steps)
         print(json.dumps({"workload": "pillars2048", "dense_kernel_ms": p["dense"]["kern_ms"],
-                          "skip_ms_per_tick": p["skip"]["kern_ms"], "hash": p["dense"]["hash"]}), flush=True)
+                          "fused_ms_per_tick": p["fused"]["kern_ms"], "skip_ms_per_tick": p["skip"]["kern_ms"],
+                          "hash": p["dense"]["hash"]}), flush=True)
         return
     if args.only == "copy":
         c = bench_copy(F, 8 << 30)
@@ -372,6 +378,11 @@ def main():
             "dense_gvox_per_s": round(p["vox"] * p["ticks"] / pd["wall"] / 1e9, 2),
             "dense_kernel_ms": round(pd["kern_ms"], 4), "dense_achieved_GBps": round(ach2 / 1e9, 1),
             "dense_hbm_frac": round(frac2, 4), "dense_frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
+            "fused_pairs": {"kernel": "tick2_kernel<32> (two dense ticks per pass, first tick's rows in LDS)",
+                            "gvox_per_s": round(p["vox"] * p["ticks"] / p["fused"]["wall"] / 1e9, 2),
+                            "ms_per_tick": round(p["fused"]["kern_ms"], 4),
+                            "achieved_GBps": round(p["vox"] * 2 / (2 * p["fused"]["kern_ms"] * 1e-3) / 1e9, 1),
+                            "note": "achieved counts the 2 bytes per voxel that one pass moves for two ticks"},
             "final_hash": pd["hash"], "segment_bytes": pd["seg"]}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4)}
