@@ -105,6 +105,16 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
 // Resident 256-thread workgroups per compute unit of the listed tick's
 // persistent kernel for a segment size.
 int list_blocks_per_cu(int seg);
+// Skipping tick that packs the active lane segments of the listed row quads
+// into full waves: the listed tick's plan pass, then a persistent pass of
+// pack_waves waves (a multiple of 4) over the list, `chunk` (<= 64) entries per
+// wave and pass.  No hash.  Same list, capacity and count slots as the listed
+// tick.
+hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
+                              const u64 *tick_dev, u32 k_off, const u64 *rule, hipStream_t s, u64 *const *masks,
+                              u64 *stats, TickEntry *list, u32 *ctl, size_t mask_words, u32 pack_waves,
+                              int chunk);
+int pack_blocks_per_cu(int seg);
 hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s);
 // Two dense ticks (t and t + 1) in one pass: src -> dst holds the state after
 // both.  Supported for one row quad per wave and LDS tiles within 64 KiB.
@@ -169,6 +179,7 @@ struct fs3d_world {
   fs3d::TickEntry *d_list = nullptr;  // work list of the listed tick (mask_words entries)
   fs3d::u32 *d_ctl = nullptr;         // list counts, one per tick mod 3
   fs3d::u32 list_waves = 0;           // persistent waves of the listed tick
+  fs3d::u32 pack_waves = 0;           // persistent waves of the packed tick
   // cached graphs keyed by (nticks, hash_every, cur)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 };

@@ -99,8 +99,9 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
 
 // ---- listed (work-list) skipping tick ------------------------------------------
 // A plan pass turns the change masks into the lane-activity word of every row
-// quad, (mine | o | o << 1 | o >> 1) over the 64 lane bits, o being the OR of
-// the four other-parity masks that share its rows; row quads with no active lane
+// quad, (mine | o | o << 1 | o >> 1) over the 64 lane bits on odd ticks and
+// (mine | o) on even ticks, o being the OR of the four other-parity masks that
+// share its rows; row quads with no active lane
 // are settled (their mask is already 0 and stays so) and cost nothing more, the
 // others are appended to a work list.  A persistent kernel then streams the list.
 
@@ -139,7 +140,8 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
           const int yo = yq[i] + dy - p, zo = zq[i] + dz - p;
           if (yo >= 0 && yo < nyq_o && zo >= 0 && zo < nzq_o) o |= other[(long long)zo * nyq_o + yo];
         }
-      act[i] = (a.masks[p][q] | o | (o << 1) | (o >> 1)) & lane_bits;
+      // the neighbours' bits matter on odd ticks only (straddling blocks)
+      act[i] = (a.masks[p][q] | o | (p ? (o << 1) | (o >> 1) : 0ull)) & lane_bits;
     }
     bal[i] = __ballot(act[i] != 0);
     if (lane == 0) wcount[i * 4 + wv] = (u32)__builtin_popcountll(bal[i]);
@@ -224,6 +226,136 @@ __global__ __launch_bounds__(256) void tick_list_kernel(TickArgs a, const TickEn
         atomicAdd((unsigned long long *)(a.stats + (size_t)(wave % kHashParts) * kHashStride),
                   (unsigned long long)swept);
     }
+  }
+}
+
+// ---- packed skipping tick -------------------------------------------------------
+// The listed tick gives each listed row quad a whole wave, which idles most lanes
+// when activity is sparse within rows, and a wave pays for the block update
+// whatever its active lanes.  This kernel reads the same list (same plan pass),
+// a chunk of up to 64 entries per wave, and packs their active lane segments
+// into full waves of 64 items: item
+// i of the chunk is the k-th active segment of entry o, with o found by a binary
+// search over the lanes' inclusive prefix of popcounts and the segment by a
+// select of the k-th set bit.  Items are in (entry, segment) order, so a lane's
+// left neighbour segment is active exactly when the entry's bit seg-1 is set,
+// and then it sits in the previous lane (or, for lane 0, in the previous group:
+// its edge bytes are read from memory).  On odd ticks the straddling block takes
+// the neighbour's edge bytes from the adjacent lane (DPP) or memory; next to a
+// settled segment it keeps its input, as in the other skipping schedules.  Change
+// bits are ORed per entry within a group (segmented scan) and routed to the
+// entry's lane, which writes the entry's mask word after the last group.
+
+template <int S>
+__global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEntry *__restrict__ list,
+                                                        const u32 *ctl, u32 nwaves, int chunk) {
+  const int lane = threadIdx.x & 63;
+  const u32 wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const u64 t64 = tick_of(a);
+  const u32 t = (u32)t64;
+  const int p = (int)(t & 1u);
+  const u64 count = ctl[t64 % 3];
+  const int LR = a.lanes_per_row;
+  const int nyq = (a.ny + p + 1) >> 1;
+  const u64 lane_bits = LR == 64 ? ~0ull : ((1ull << LR) - 1ull);
+  u64 *mine = a.masks[p];
+  for (u64 c0 = (u64)wave * chunk; c0 < count; c0 += (u64)nwaves * chunk) {
+    // this lane's entry of the chunk, and the prefix of active segments
+    const u64 e = c0 + lane;
+    const bool has = lane < chunk && e < count;
+    const u64 act = has ? list[e].act : 0ull;
+    const u64 pos = has ? list[e].pos : 0ull;
+    const int n = __builtin_popcountll(act);
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    const int excl = incl - n;
+    const int total = __shfl(incl, 63, 64);
+    u64 acc = 0;  // change bits of this lane's entry
+    for (int g0 = 0; g0 < total; g0 += 64) {
+      // An opaque copy of the item index: otherwise predicates derived from the
+      // lane index are hoisted out of the loop into SGPR pairs.
+      int idx = g0 + lane;
+      asm volatile("" : "+v"(idx));
+      const bool active = idx < total;
+      // owner: the number of entries whose inclusive prefix is <= idx
+      int o = 0;
+#pragma unroll
+      for (int w = 32; w >= 1; w >>= 1) {
+        const int v = __shfl(incl, o + w - 1, 64);
+        if (v <= idx) o += w;
+      }
+      o = active ? o : 63;
+      int k = idx - __shfl(excl, o, 64);
+      const u64 ao = ((u64)(u32)__shfl((int)(act >> 32), o, 64) << 32) | (u32)__shfl((int)(u32)act, o, 64);
+      const u32 yq = (u32)__shfl((int)(u32)pos, o, 64), zq = (u32)__shfl((int)(pos >> 32), o, 64);
+      // the k-th set bit of ao
+      int seg = 0;
+#pragma unroll
+      for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __builtin_popcountll((ao >> seg) & ((1ull << w) - 1ull));
+        if (k >= c) {
+          k -= c;
+          seg += w;
+        }
+      }
+      seg = active ? seg : 0;
+      const bool left_act = active && seg > 0 && ((ao >> (seg - 1)) & 1ull);
+      const bool right_act = active && seg < LR - 1 && ((ao >> (seg + 1)) & 1ull);
+      RowQuad<S> R;
+      R.seg = seg;
+      R.active = active;
+      R.yq = (int)yq;
+      R.zq = (int)zq;
+      R.q = (long long)zq * nyq + yq;
+      R.skip = false;
+      R.nl = active && seg > 0 && !left_act;
+      R.nr = active && seg < LR - 1 && !right_act;
+      R.ml = left_act && lane == 0;
+      R.mr = right_act && lane == 63;
+      rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, true);
+#pragma unroll
+      for (int r = 0; r < 4; r++) R.edge[r] = 0;
+      if (p && (R.ml || R.mr)) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (R.valid >> r & 1) {
+            const u8 *row = a.src + R.rowoff[r];
+            const u32 l = R.ml ? row[-1] : 0u;
+            const u32 rt = R.mr ? row[S] : 0u;
+            R.edge[r] = (l << 24) | rt;
+          } else {
+            R.edge[r] = kStatic4;
+          }
+        }
+      }
+      bool ch = false;
+      rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch);
+      // OR of the change bits of each entry's run of lanes in this group ...
+      u64 bits = (active && ch) ? (1ull << seg) : 0ull;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const u64 ob = (u64)__shfl_up((unsigned long long)bits, d, 64);
+        const int oo = __shfl_up(o, d, 64);
+        if (lane >= d && oo == o) bits |= ob;
+      }
+      // ... read by the entry's lane from the run's last lane
+      const int s0 = max(excl, g0), s1 = min(incl, g0 + 64);
+      const int src = s1 > s0 ? s1 - 1 - g0 : lane;
+      const u64 run = (u64)__shfl((unsigned long long)bits, src, 64);
+      if (s1 > s0) acc |= run;
+      if (a.stats) {
+        const u64 vox = active ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
+        const u64 swept = wave_sum_u64(vox);
+        if (lane == 0 && swept)
+          atomicAdd((unsigned long long *)(a.stats + (size_t)(wave % kHashParts) * kHashStride),
+                    (unsigned long long)swept);
+      }
+    }
+    if (has) mine[(long long)(u32)(pos >> 32) * nyq + (u32)pos] = acc & lane_bits;
   }
 }
 
@@ -391,6 +523,51 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
     default: hipLaunchKernelGGL((tick_list_kernel<64>), grid, dim3(256), 0, s, a, list, ctl, list_waves); break;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
+                              const u64 *tick_dev, u32 k_off, const u64 *rule, hipStream_t s, u64 *const *masks,
+                              u64 *stats, TickEntry *list, u32 *ctl, size_t mask_words, u32 pack_waves,
+                              int chunk) {
+  TickArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.lanes_per_row = sh.lanes_per_row;
+  a.rq_per_wave = sh.rq_per_wave;
+  a.t = t;
+  a.k_off = k_off;
+  a.tick_dev = tick_dev;
+  a.rule = rule;
+  a.parts = nullptr;
+  a.masks[0] = masks[0];
+  a.masks[1] = masks[1];
+  a.stats = stats;
+  a.ctl = ctl;
+  const size_t plan_blocks = (mask_words + 256 * kPlanPerThread - 1) / (256 * kPlanPerThread);
+  hipLaunchKernelGGL(plan_kernel, dim3((unsigned)plan_blocks), dim3(256), 0, s, a, list, ctl);
+  const dim3 grid(pack_waves / 4);
+  switch (sh.seg) {
+    case 8: hipLaunchKernelGGL((tick_pack_kernel<8>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    case 16: hipLaunchKernelGGL((tick_pack_kernel<16>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    case 32: hipLaunchKernelGGL((tick_pack_kernel<32>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    default: hipLaunchKernelGGL((tick_pack_kernel<64>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+  }
+  return hipGetLastError();
+}
+
+int pack_blocks_per_cu(int seg) {
+  int n = 0;
+  hipError_t e;
+  switch (seg) {
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<8>, 256, 0); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<16>, 256, 0); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<32>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<64>, 256, 0); break;
+  }
+  return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 int list_blocks_per_cu(int seg) {

@@ -142,9 +142,9 @@ __device__ __forceinline__ u32 pack_states(u32 c0, u32 c1, u32 c2, u32 c3) {
 // destination buffer, which received tick t-2's output, already holds its bytes.
 // Changes are recorded per parity and row quad as a mask over lane segments: bit s
 // is set when lane s evaluated a block that changed.  At tick t (parity p) a lane
-// reads its own bit from the parity-p masks (written at tick t-2) and bits s-1..s+1
-// of the four parity-(1-p) row quads that share its rows (tick t-1), then rewrites
-// its row quad's parity-p mask.  Host-side edits set the bits of the cells they
+// reads its own bit from the parity-p masks (written at tick t-2) and, of the four
+// parity-(1-p) row quads that share its rows (tick t-1), bits s-1..s+1 on odd
+// ticks and bit s on even ticks, then rewrites its row quad's parity-p mask.  Host-side edits set the bits of the cells they
 // touch (or every bit), so edited cells are recomputed at both parities.
 struct SkipCtx {
   u64 *mine;          // parity-p masks: read (tick t-2), then rewritten (tick t)
@@ -166,6 +166,12 @@ struct RowQuad {
   bool active;  // the lane holds a row quad of this tick
   bool skip;    // settled: nothing to evaluate or store
   bool loaded;  // own[] holds the grid bytes (false: skipped without loading)
+  // Segment-list tick only (rq_update's EDGES): the left / right neighbour
+  // segment is settled (nl / nr), or active but held by another wave, its edge
+  // byte then read from memory into edge[] (ml / mr; edge[r] = left byte << 24 |
+  // right byte).
+  bool nl, nr, ml, mr;
+  u32 edge[4];
 };
 
 // Loads the lane's segment of the 4 rows of row quad (R.yq, R.zq) at parity p
@@ -219,8 +225,13 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
           const int yo = R.yq + dy - p, zo = R.zq + dz - p;
           if (yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o) o |= sk->other[(long long)zo * sk->nyq_o + yo];
         }
-      // segments seg, seg+1 and seg-1 at tick t-1 (shifted right only: seg may be 63)
-      const u64 near = ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u);
+      // Odd ticks: segments seg, seg+1 and seg-1 at tick t-1 (the straddling
+      // blocks read a byte of each neighbour; shifted right only: seg may be 63).
+      // Even ticks: segment seg alone.  No block straddles a segment edge at even
+      // parity, and every block that touched one of this segment's cells at the
+      // odd tick t-1 was evaluated by this segment's lane there, so its bit holds
+      // every change to them.
+      const u64 near = p ? ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u) : (o >> R.seg) & 1u;
       const u64 own = (sk->mine[q] >> R.seg) & 1u;  // this segment at tick t-2
       R.skip = (near | own) == 0;
     }
@@ -233,7 +244,9 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // lane of the wave must call it together (DPP exchange).
 // With SKIP or TRACK, *changed reports whether a block this lane evaluated
 // changed.
-template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP>
+// With EDGES (segment-list tick), a lane's neighbours come from R.nl / nr / ml /
+// mr / edge instead of the adjacent lanes' load state.
+template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false>
 __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
                                          bool *changed = nullptr) {
   constexpr int ND = S / 4;  // dwords per lane per row
@@ -253,8 +266,12 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
   u32 fr[4][NF];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const u32 from_l = lane_from_left(R.own[r][ND - 1]);
-    const u32 from_r = lane_from_right(R.own[r][0]);
+    u32 from_l = lane_from_left(R.own[r][ND - 1]);
+    u32 from_r = lane_from_right(R.own[r][0]);
+    if constexpr (EDGES) {  // the adjacent lane may hold an unrelated segment
+      from_l = R.ml ? R.edge[r] : (R.nl ? kStatic4 : from_l);
+      from_r = R.mr ? R.edge[r] : (R.nr ? kStatic4 : from_r);
+    }
     if (p) {
       const u32 lb = seg == 0 ? 0x01000000u : (from_l & 0xFF000000u);
       // frame bytes 2-3 of the last dword hold no block: STATIC, like the outputs
@@ -274,7 +291,10 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
   // Their edge bytes are placeholders; the block straddling into such a
   // neighbour is settled (its own bit covered it), so it keeps its input.
   bool nl = false, nr = false;
-  if constexpr (SKIP) {
+  if constexpr (SKIP && EDGES) {
+    nl = p && R.nl;
+    nr = p && R.nr;
+  } else if constexpr (SKIP) {
     const u32 unl = R.loaded ? 0u : 1u;
     const u32 fl = lane_from_left(unl), fr_ = lane_from_right(unl);
     nl = p && seg > 0 && fl != 0;

@@ -22,7 +22,7 @@ EMPTY, STATIC, SAND, FLUID = 0, 1, 2, 3
 BOXES_512 = dict(side_min=8, side_max=64, pct=(20, 10, 5), max_boxes=1 << 20)
 PILLARS_2048 = (256, 4, 32, 256, 1800, 256, 682, 256, 682, 32, 128)
 DENSE = 1  # world flag: settled-segment skipping off
-SKIP_OFF, SKIP_AUTO, SKIP_LISTED, SKIP_WAVES = 0, 1, 2, 3
+SKIP_OFF, SKIP_AUTO, SKIP_LISTED, SKIP_WAVES, SKIP_SEGS = 0, 1, 2, 3, 4
 
 
 def device_count():
@@ -232,7 +232,7 @@ class World:
     @property
     def skip(self):
         """Settled-segment skipping mode: SKIP_OFF, SKIP_AUTO (the default unless
-        the world was created with DENSE), SKIP_LISTED or SKIP_WAVES.  Setting
+        the world was created with DENSE), SKIP_LISTED, SKIP_WAVES or SKIP_SEGS.  Setting
         True / False means SKIP_AUTO / SKIP_OFF."""
         s = c_i32(0)
         check(lib().fs3d_world_get_skip(self.handle, ctypes.byref(s)))

@@ -109,16 +109,19 @@ int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg_bytes);
  * segment (S bytes of 4 rows) unread and unwritten when none of the blocks it
  * evaluates changed at the previous tick of the same parity and none of their
  * cells changed at the tick before; results are bit-identical to the dense tick
- * (whether a block is a fixed point does not depend on its variant).  Two
+ * (whether a block is a fixed point does not depend on its variant).  Three
  * schedules: LISTED runs a plan pass that lists the row quads with work and a
- * persistent pass over the list (settled regions cost nothing; best for large
- * grids), WAVES launches one wave per row-quad group that skips on its own
- * (lower per-tick overhead; best for small grids).  AUTO picks LISTED above
- * 2^17 waves of work per tick.  Hash ticks read settled segments in either.
+ * persistent pass over the list (settled regions cost nothing); SEGS reads the
+ * same list and packs the listed row quads' active lane segments 64 to a wave
+ * (the block update is paid per active segment, not per row quad with one);
+ * WAVES launches one wave per row-quad group that skips on its own (lower
+ * per-tick overhead; best for small grids).  AUTO picks SEGS above 2^17 waves
+ * of work per tick, else WAVES.  Hash ticks read settled segments in all
+ * three.
  * Every host-side edit (set_voxel, upload, fills, set_tick, set_segment) keeps
  * the bookkeeping exact; writing the grid through fs3d_world_device_ptr is not
  * supported. */
-enum { FS3D_SKIP_OFF = 0, FS3D_SKIP_AUTO = 1, FS3D_SKIP_LISTED = 2, FS3D_SKIP_WAVES = 3 };
+enum { FS3D_SKIP_OFF = 0, FS3D_SKIP_AUTO = 1, FS3D_SKIP_LISTED = 2, FS3D_SKIP_WAVES = 3, FS3D_SKIP_SEGS = 4 };
 int fs3d_world_set_skip(fs3d_world *w, int32_t mode);
 int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode);
 /* Two-tick fusion of the dense tick (used when skipping is off): a pair of

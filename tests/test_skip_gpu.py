@@ -9,9 +9,10 @@ from conftest import announce
 
 pytestmark = pytest.mark.gpu
 
-# both schedules: a plan pass plus a persistent pass over the listed row quads,
-# and one wave per row-quad group skipping on its own
-MODES = pytest.mark.parametrize("mode", [2, 3], ids=["listed", "waves"])
+# the three schedules: a plan pass plus a persistent pass over the listed row
+# quads, one wave per row-quad group skipping on its own, and a plan pass plus a
+# persistent pass over the listed lane segments
+MODES = pytest.mark.parametrize("mode", [2, 3, 4], ids=["listed", "waves", "segs"])
 
 
 def _scene(rng, shape, fluid=True):
