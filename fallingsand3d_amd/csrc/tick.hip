@@ -21,6 +21,7 @@ struct TickArgs {
   u8 *dst;
   int nx, ny, nz;
   int lanes_per_row, rq_per_wave;
+  u32 lr_magic;  // ceil(65536 / lanes_per_row): lane / lanes_per_row = (lane * lr_magic) >> 16
   u64 t;
   u32 k_off;
   const u64 *tick_dev;  // when set, tick = *tick_dev + k_off (graph replays)
@@ -33,21 +34,26 @@ struct TickArgs {
 
 __device__ __forceinline__ u64 tick_of(const TickArgs &a) { return a.tick_dev ? *a.tick_dev + a.k_off : a.t; }
 
+// The grid is x: groups of 4 * rq_per_wave row quads along y (one group per
+// workgroup), y and z: z row quads (zq = blockIdx.z * 65535 + blockIdx.y), so a
+// lane's row quad comes from the block indices without a division.
 template <int S, bool HASH, bool SKIP>
 __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
-  const int lane = threadIdx.x & 63;
-  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const u64 t64 = tick_of(a);
   const u32 t = (u32)t64;
   const int p = (int)(t & 1u);
-  const long long nrq = (long long)((a.ny + p + 1) >> 1) * ((a.nz + p + 1) >> 1);
+  const int nyq = (a.ny + p + 1) >> 1, nzq = (a.nz + p + 1) >> 1;
+  const int LR = a.lanes_per_row, rqpw = a.rq_per_wave;
+  const int rql = (int)(((u32)lane * a.lr_magic) >> 16);
+  const int seg = lane - rql * LR;
+  const int yq = ((int)blockIdx.x * 4 + wv) * rqpw + rql;
+  const int zq = (int)blockIdx.z * 65535 + (int)blockIdx.y;
+  const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const long long wave = bid * 4 + wv;
   // A hash tick replaces the plan pass of the listed tick: it clears the count
   // slot the next tick's plan pass appends to.
-  if (SKIP && HASH && a.ctl && blockIdx.x == 0 && threadIdx.x == 0) a.ctl[(t64 + 1) % 3] = 0;
-  // Whole waves past the end leave early; the hashing variant keeps them until
-  // its workgroup barrier (they load and store nothing).
-  if (!HASH && wave * a.rq_per_wave >= nrq) return;
-  RowQuad<S> R;
+  if (SKIP && HASH && a.ctl && bid == 0 && threadIdx.x == 0) a.ctl[(t64 + 1) % 3] = 0;
   SkipCtx sk;
   if constexpr (SKIP) {
     sk.mine = a.masks[p];
@@ -56,8 +62,21 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
     sk.nzq_o = (a.nz + (p ^ 1) + 1) >> 1;
     sk.load_skipped = HASH;  // hash ticks read settled segments to hash them
   }
-  rq_load<S, SKIP>(R, a.src, a.nx, a.ny, a.nz, a.lanes_per_row, a.rq_per_wave, p, wave, lane, &sk);
-  const u64 lane_bits = a.lanes_per_row == 64 ? ~0ull : ((1ull << a.lanes_per_row) - 1ull);
+  const u64 lane_bits = LR == 64 ? ~0ull : ((1ull << LR) - 1ull);
+  // Whole waves past the end of the row leave early; the hashing variant keeps
+  // them until its workgroup barrier (they load and store nothing).
+  const bool active = rql < rqpw && yq < nyq && zq < nzq;
+  if (!HASH && (__builtin_amdgcn_readfirstlane((int)(((int)blockIdx.x * 4 + wv) * rqpw)) >= nyq || zq >= nzq))
+    return;
+  RowQuad<S> R;
+  R.seg = seg;
+  R.active = active;
+  R.yq = active ? yq : 0;
+  R.zq = active ? zq : 0;
+  R.q = (long long)R.zq * nyq + R.yq;
+  R.skip = false;
+  if constexpr (SKIP) rq_skip_check(R, p, &sk);
+  rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, !SKIP || !R.skip || sk.load_skipped);
   if constexpr (SKIP && !HASH) {
     if (__ballot(R.active && !R.skip) == 0) {  // the whole wave is settled
       if (R.active && R.seg == 0) sk.mine[R.q] = 0;
@@ -65,7 +84,7 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
     }
   }
   bool ch = false;
-  u64 acc = rq_update<S, HASH, SKIP>(R, a.dst, a.lanes_per_row, t, a.rule, &ch);
+  u64 acc = rq_update<S, HASH, SKIP>(R, a.dst, LR, t, a.rule, &ch);
   if constexpr (SKIP) {
     const u64 b = __ballot(ch);
     if (R.active && R.seg == 0) sk.mine[R.q] = (b >> (lane - R.seg)) & lane_bits;
@@ -91,7 +110,7 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) {
       const u64 tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-      if (tot) atomicAdd((unsigned long long *)(a.parts + (size_t)(blockIdx.x % kHashParts) * kHashStride),
+      if (tot) atomicAdd((unsigned long long *)(a.parts + (size_t)(bid % kHashParts) * kHashStride),
                          (unsigned long long)tot);
     }
   }
@@ -415,14 +434,20 @@ __global__ __launch_bounds__(256) void counts_kernel(const u8 *g, size_t n, u64 
 
 template <int S>
 hipError_t launch_s(const TickShape &sh, const TickArgs &a, bool hash, bool skip, hipStream_t s) {
+  // sized for the larger parity's row-quad counts (graph replays read the tick
+  // from the device); the kernel bounds itself by the tick's own
+  const int nyq_max = (a.ny + 2) >> 1, nzq_max = (a.nz + 2) >> 1;
+  const int per_block = 4 * sh.rq_per_wave;
+  const dim3 grid((unsigned)((nyq_max + per_block - 1) / per_block), (unsigned)std::min(nzq_max, 65535),
+                  (unsigned)((nzq_max + 65534) / 65535));
   if (hash && skip)
-    hipLaunchKernelGGL((tick_kernel<S, true, true>), dim3(sh.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tick_kernel<S, true, true>), grid, dim3(256), 0, s, a);
   else if (hash)
-    hipLaunchKernelGGL((tick_kernel<S, true, false>), dim3(sh.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tick_kernel<S, true, false>), grid, dim3(256), 0, s, a);
   else if (skip)
-    hipLaunchKernelGGL((tick_kernel<S, false, true>), dim3(sh.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tick_kernel<S, false, true>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((tick_kernel<S, false, false>), dim3(sh.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tick_kernel<S, false, false>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -473,6 +498,7 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
   a.nz = nz;
   a.lanes_per_row = sh.lanes_per_row;
   a.rq_per_wave = sh.rq_per_wave;
+  a.lr_magic = (65536u + (u32)sh.lanes_per_row - 1u) / (u32)sh.lanes_per_row;
   a.t = t;
   a.k_off = k_off;
   a.tick_dev = tick_dev;
@@ -504,6 +530,7 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.nz = nz;
   a.lanes_per_row = sh.lanes_per_row;
   a.rq_per_wave = sh.rq_per_wave;
+  a.lr_magic = (65536u + (u32)sh.lanes_per_row - 1u) / (u32)sh.lanes_per_row;
   a.t = t;
   a.k_off = k_off;
   a.tick_dev = tick_dev;
@@ -537,6 +564,7 @@ hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.nz = nz;
   a.lanes_per_row = sh.lanes_per_row;
   a.rq_per_wave = sh.rq_per_wave;
+  a.lr_magic = (65536u + (u32)sh.lanes_per_row - 1u) / (u32)sh.lanes_per_row;
   a.t = t;
   a.k_off = k_off;
   a.tick_dev = tick_dev;

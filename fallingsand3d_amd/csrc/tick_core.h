@@ -200,6 +200,30 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, i
   }
 }
 
+// Sets R.skip for an active lane from the change masks (R.seg, R.yq, R.zq, R.q
+// set).
+template <int S>
+__device__ __forceinline__ void rq_skip_check(RowQuad<S> &R, int p, const SkipCtx *sk) {
+  if (!R.active) return;
+  u64 o = 0;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+      const int yo = R.yq + dy - p, zo = R.zq + dz - p;
+      if (yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o) o |= sk->other[(long long)zo * sk->nyq_o + yo];
+    }
+  // Odd ticks: segments seg, seg+1 and seg-1 at tick t-1 (the straddling
+  // blocks read a byte of each neighbour; shifted right only: seg may be 63).
+  // Even ticks: segment seg alone.  No block straddles a segment edge at even
+  // parity, and every block that touched one of this segment's cells at the
+  // odd tick t-1 was evaluated by this segment's lane there, so its bit holds
+  // every change to them.
+  const u64 near = p ? ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u) : (o >> R.seg) & 1u;
+  const u64 own = (sk->mine[R.q] >> R.seg) & 1u;  // this segment at tick t-2
+  R.skip = (near | own) == 0;
+}
+
 // Loads row-quad group `wave` (in units of rqpw row quads) of a tick of parity p.
 template <int S, bool SKIP = false>
 __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int LR, int rqpw,
@@ -215,27 +239,7 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
   R.q = q;
   R.active = active;
   R.skip = false;
-  if constexpr (SKIP) {
-    if (active) {
-      u64 o = 0;
-#pragma unroll
-      for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-        for (int dy = 0; dy < 2; dy++) {
-          const int yo = R.yq + dy - p, zo = R.zq + dz - p;
-          if (yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o) o |= sk->other[(long long)zo * sk->nyq_o + yo];
-        }
-      // Odd ticks: segments seg, seg+1 and seg-1 at tick t-1 (the straddling
-      // blocks read a byte of each neighbour; shifted right only: seg may be 63).
-      // Even ticks: segment seg alone.  No block straddles a segment edge at even
-      // parity, and every block that touched one of this segment's cells at the
-      // odd tick t-1 was evaluated by this segment's lane there, so its bit holds
-      // every change to them.
-      const u64 near = p ? ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u) : (o >> R.seg) & 1u;
-      const u64 own = (sk->mine[q] >> R.seg) & 1u;  // this segment at tick t-2
-      R.skip = (near | own) == 0;
-    }
-  }
+  if constexpr (SKIP) rq_skip_check(R, p, sk);
   rq_fetch<S>(R, src, nx, ny, nz, p, !SKIP || !R.skip || sk->load_skipped);
 }
 
