@@ -397,10 +397,19 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
           if (k < R.m) dp[k] = (u8)(res[k >> 2] >> (8 * (k & 3)));
       }
       if constexpr (HASH) {
+        // Each non-empty cell costs a 64-bit mix; a dword position that is
+        // empty in every lane of the wave is skipped as a whole (a uniform
+        // branch), which spares most of the work in empty space.
 #pragma unroll
-        for (int k = 0; k < S; k++) {
-          const u32 v = (res[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-          if (k < R.m && v) acc += dmix64(((u64)(R.rowoff[r] + k) << 8) | v);
+        for (int j = 0; j < ND; j++) {
+          if (__any(res[j] != 0u)) {
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+              const int k = 4 * j + kk;
+              const u32 v = (res[j] >> (8 * kk)) & 0xFFu;
+              if (k < R.m && v) acc += dmix64(((u64)(R.rowoff[r] + k) << 8) | v);
+            }
+          }
         }
       }
     }
