@@ -199,6 +199,46 @@ int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, do
   return clear_error();
 }
 
+int fs3d_key_camera(const int32_t dims[3], const float cam_pos[3], const float cam_rot[3], float aspect,
+                    fs3d_camera *out) {
+  if (!dims || !cam_pos || !cam_rot || !out || dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || !(aspect > 0.0f))
+    return set_error(FS3D_ERR_INVALID, "key camera: bad arguments");
+  // the reference's view matrix, as in fs3d_camera_globaldata
+  const M4 view = mul(mul(rotation(radians(0.0f), 1.0f, 0.0f, 0.0f), rotation(radians(cam_rot[1]), 0.0f, 1.0f, 0.0f)),
+                      translation(cam_pos[0], cam_pos[1], cam_pos[2]));
+  // Camera axes in world space are the rows of the rotation part; the eye is
+  // -R^T t.  The camera looks down -z with +y up on screen (the projection's
+  // flipped y turns Vulkan's downward NDC y back up), so screen-left = -row 0,
+  // screen-down = -row 1 and forward = -row 2.
+  double eye[3], axis[3][3];
+  for (int j = 0; j < 3; j++) {
+    eye[j] = 0.0;
+    for (int i = 0; i < 3; i++) eye[j] -= (double)view.c[j][i] * view.c[3][i];
+    for (int k = 0; k < 3; k++) axis[k][j] = -(double)view.c[j][k];
+  }
+  // World -> grid: the grid spans [-1, 1] along its longest axis around the
+  // origin; world +X = grid +x, world +Y = grid +z (up), world +Z = grid -y.
+  const double half = 0.5 * std::max(std::max((double)dims[0], (double)dims[1]), (double)dims[2]);
+  const double c[3] = {dims[0] * 0.5, dims[1] * 0.5, dims[2] * 0.5};
+  auto to_grid = [](const double v[3], double g[3]) {
+    g[0] = v[0];
+    g[1] = -v[2];
+    g[2] = v[1];
+  };
+  double g[3];
+  to_grid(eye, g);
+  for (int a = 0; a < 3; a++) out->origin[a] = (float)(c[a] + half * g[a]);
+  for (int k = 0; k < 3; k++) {
+    to_grid(axis[k], g);
+    for (int a = 0; a < 3; a++) out->basis[3 * k + a] = (float)g[a];
+  }
+  const double light[3] = {2.0, 5.0, 3.0};  // fs_raymarch's light position (shaders/fs_raymarch.frag:51)
+  to_grid(light, g);
+  for (int a = 0; a < 3; a++) out->light[a] = (float)(c[a] + half * g[a]);
+  out->aspect = aspect;
+  return clear_error();
+}
+
 int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, float *rgba) {
   if (!cam_pos || !rgba || W < 1 || H < 1 || !(aspect > 0.0f))
     return set_error(FS3D_ERR_INVALID, "render_sdf: bad arguments");

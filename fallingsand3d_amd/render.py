@@ -54,6 +54,16 @@ def orbit_camera(dims, angle, dist_scale=1.6, elev=0.4, aspect=16.0 / 9.0):
     return cam
 
 
+def key_camera(dims, cam_pos, cam_rot, aspect=16.0 / 9.0):
+    """Voxel camera for the reference's key-driven camera state (see camera_integrate)."""
+    d = (c_i32 * 3)(*[int(v) for v in dims])
+    p = np.ascontiguousarray(cam_pos, np.float32)
+    r = np.ascontiguousarray(cam_rot, np.float32)
+    cam = Camera()
+    check(lib().fs3d_key_camera(d, ptr(p, c_float), ptr(r, c_float), aspect, ctypes.byref(cam)))
+    return cam
+
+
 def camera(origin, basis, light, aspect):
     cam = Camera()
     cam.origin[:] = [float(v) for v in origin]

@@ -248,6 +248,17 @@ int fs3d_world_last_render_ms(const fs3d_world *w, double *ms);
  * dist_scale * max(dims) away, looking at the centre with +z up on screen. */
 int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, double elev, float aspect,
                       fs3d_camera *out);
+/* Voxel camera from the reference's key-driven camera state (cam_pos, cam_rot
+ * as fs3d_camera_integrate advances them; the view matrix of
+ * fs3d_camera_globaldata, src/engine/rendering/materials.cpp:532-536): eye =
+ * -R^T t, screen-left / screen-down / forward = minus rows 0 / 1 / 2 of the
+ * view's rotation.  World units map to the grid so that [-1, 1] spans its
+ * longest axis around its centre, world +X = grid +x, +Y = grid +z (up),
+ * +Z = grid -y; the light is fs_raymarch's (2, 5, 3) mapped the same way.  The
+ * reference's initial camera (0, 0, -5) looks at the grid from 5 half-extents
+ * along -y. */
+int fs3d_key_camera(const int32_t dims[3], const float cam_pos[3], const float cam_rot[3], float aspect,
+                    fs3d_camera *out);
 /* The reference's GlobalData block (src/engine/rendering/materials.h:19-28) as
  * Material::writeGlobalAndObjectData fills it (src/engine/rendering/materials.cpp:531-545):
  * 76 floats = camPos, aspect, camDir (vec4 each), view, proj, viewproj,
