@@ -250,11 +250,13 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // changed.
 // With EDGES (segment-list tick), a lane's neighbours come from R.nl / nr / ml /
 // mr / edge instead of the adjacent lanes' load state.
-template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false>
-__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
-                                         bool *changed = nullptr) {
+// P = t & 1 when known at compile time (even ticks need no neighbour exchange,
+// no shifted frame and no extra frame dword), or -1: the parity is read from t.
+template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P>
+__device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
+                                           bool *changed) {
   constexpr int ND = S / 4;  // dwords per lane per row
-  const int p = (int)(t & 1u);
+  const int p = P < 0 ? (int)(t & 1u) : P;
   const int seg = R.seg;
 
   // ---- reserved bytes: wave-uniform slow path ----
@@ -417,6 +419,24 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
   return acc;
 }
 
+// One copy of the update per parity, chosen by a branch that is uniform: the
+// tick's parity is the same across a wave in every caller (kernel arguments, or
+// one world per workgroup).
+template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false>
+__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
+                                         bool *changed = nullptr) {
+  if (t & 1u) return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 1>(R, dst, LR, t, rule, changed);
+  return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 0>(R, dst, LR, t, rule, changed);
+}
+
+// A single copy for both parities, for the batched kernels' persistent loops
+// (two copies there double the live registers).
+template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP>
+__device__ __forceinline__ u64 rq_update_any(const RowQuad<S> &R, u8 *dst, int LR, u32 t,
+                                             const u64 *__restrict__ rule, bool *changed = nullptr) {
+  return rq_update_p<S, HASH, SKIP, TRACK, false, -1>(R, dst, LR, t, rule, changed);
+}
+
 // Processes row-quad group `wave` of one tick at tick counter t (src may equal
 // dst).  Every lane of the wave must call it.
 // With TRACK, *changed reports whether a block of this lane changed.
@@ -426,7 +446,7 @@ __device__ __forceinline__ u64 tick_wave(const u8 *src, u8 *dst, int nx, int ny,
                                          bool *changed = nullptr) {
   RowQuad<S> R;
   rq_load<S>(R, src, nx, ny, nz, LR, rqpw, (int)(t & 1u), wave, lane);
-  return rq_update<S, HASH, false, TRACK>(R, dst, LR, t, rule, changed);
+  return rq_update_any<S, HASH, false, TRACK>(R, dst, LR, t, rule, changed);
 }
 
 // Processes NR row-quad groups (wave0, wave0 + stride, ...) with every load
@@ -446,7 +466,7 @@ __device__ __forceinline__ u64 tick_waves(const u8 *src, u8 *dst, int nx, int ny
 #pragma unroll
   for (int i = 0; i < NR; i++) {
     bool ch = false;
-    acc += rq_update<S, HASH, false, true>(R[i], dst, LR, t, rule, &ch);
+    acc += rq_update_any<S, HASH, false, true>(R[i], dst, LR, t, rule, &ch);
     if (changed) *changed = *changed || ch;
   }
   return acc;
