@@ -151,16 +151,21 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
     if (q < nrq) {
       yq[i] = (int)(q % nyq);
       zq[i] = (int)(q / nyq);
+      // unconditional loads (out-of-range ones from word 0, discarded) overlap
+      u64 v[4];
+      bool in[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int yo = yq[i] + (k & 1) - p, zo = zq[i] + (k >> 1) - p;
+        in[k] = yo >= 0 && yo < nyq_o && zo >= 0 && zo < nzq_o;
+        v[k] = other[in[k] ? (long long)zo * nyq_o + yo : 0];
+      }
+      const u64 own = a.masks[p][q];
       u64 o = 0;
 #pragma unroll
-      for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-        for (int dy = 0; dy < 2; dy++) {
-          const int yo = yq[i] + dy - p, zo = zq[i] + dz - p;
-          if (yo >= 0 && yo < nyq_o && zo >= 0 && zo < nzq_o) o |= other[(long long)zo * nyq_o + yo];
-        }
+      for (int k = 0; k < 4; k++) o |= in[k] ? v[k] : 0ull;
       // the neighbours' bits matter on odd ticks only (straddling blocks)
-      act[i] = (a.masks[p][q] | o | (p ? (o << 1) | (o >> 1) : 0ull)) & lane_bits;
+      act[i] = (own | o | (p ? (o << 1) | (o >> 1) : 0ull)) & lane_bits;
     }
     bal[i] = __ballot(act[i] != 0);
     if (lane == 0) wcount[i * 4 + wv] = (u32)__builtin_popcountll(bal[i]);

@@ -193,26 +193,36 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, i
 #pragma unroll
     for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
     if (ok && fetch) load_seg<S>(src + R.rowoff[r], R.own[r]);
-    if (R.m < S) {
+  }
+  // a partial last segment: bytes past the row read as STATIC (after all four
+  // loads are issued, so that they overlap)
+  if (R.m < S) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
 #pragma unroll
       for (int j = 0; j < ND; j++) R.own[r][j] = keep_bytes(R.own[r][j], R.m - 4 * j);
-    }
   }
 }
 
 // Sets R.skip for an active lane from the change masks (R.seg, R.yq, R.zq, R.q
 // set).
+// The five mask loads are issued unconditionally (out-of-range ones from word 0,
+// then discarded) so that they overlap: loads under per-lane branches are each
+// waited for before the next is issued.
 template <int S>
 __device__ __forceinline__ void rq_skip_check(RowQuad<S> &R, int p, const SkipCtx *sk) {
-  if (!R.active) return;
+  u64 v[4];
+  bool in[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int yo = R.yq + (k & 1) - p, zo = R.zq + (k >> 1) - p;
+    in[k] = R.active && yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o;
+    v[k] = sk->other[in[k] ? (long long)zo * sk->nyq_o + yo : 0];
+  }
+  const u64 mine = sk->mine[R.active ? R.q : 0];
   u64 o = 0;
 #pragma unroll
-  for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-    for (int dy = 0; dy < 2; dy++) {
-      const int yo = R.yq + dy - p, zo = R.zq + dz - p;
-      if (yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o) o |= sk->other[(long long)zo * sk->nyq_o + yo];
-    }
+  for (int k = 0; k < 4; k++) o |= in[k] ? v[k] : 0ull;
   // Odd ticks: segments seg, seg+1 and seg-1 at tick t-1 (the straddling
   // blocks read a byte of each neighbour; shifted right only: seg may be 63).
   // Even ticks: segment seg alone.  No block straddles a segment edge at even
@@ -220,8 +230,8 @@ __device__ __forceinline__ void rq_skip_check(RowQuad<S> &R, int p, const SkipCt
   // odd tick t-1 was evaluated by this segment's lane there, so its bit holds
   // every change to them.
   const u64 near = p ? ((o >> R.seg) & 3u) | (R.seg > 0 ? (o >> (R.seg - 1)) & 1u : 0u) : (o >> R.seg) & 1u;
-  const u64 own = (sk->mine[R.q] >> R.seg) & 1u;  // this segment at tick t-2
-  R.skip = (near | own) == 0;
+  const u64 own = (mine >> R.seg) & 1u;  // this segment at tick t-2
+  R.skip = R.active && (near | own) == 0;
 }
 
 // Loads row-quad group `wave` (in units of rqpw row quads) of a tick of parity p.
