@@ -330,6 +330,17 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
   for (int i = 0; i < NG; i++) wg[i] = fmix32((g0 + (u32)i) * 0x9E3779B1u + yzt);
   const u32 jb = (S >= 32) ? 0u : (bx0 & 15u);
 
+  // Reserved bytes read as STATIC: the frame becomes categories in one
+  // (rarely taken) uniform branch, so that the evaluation below is one basic
+  // block in which the scheduler can issue every table gather before the first
+  // wait.
+  if (slow) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) fr[r][j] = to_codes(fr[r][j], reserved_mask(fr[r][j]));
+  }
+
   // ---- evaluate blocks (two per frame dword) and scatter outputs ----
   u32 out[4][NF];
 #pragma unroll
@@ -339,10 +350,7 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
       for (int r = 0; r < 4; r++) out[r][j] = kStatic4;
       continue;
     }
-    u32 c[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) c[r] = slow ? to_codes(fr[r][j], reserved_mask(fr[r][j])) : fr[r][j];
-    const u32 states = pack_states(c[0], c[1], c[2], c[3]);
+    const u32 states = pack_states(fr[0][j], fr[1][j], fr[2][j], fr[3][j]);
     u64 e[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -370,12 +378,15 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
     if constexpr (SKIP) {
       // The settled straddling block keeps its input: of its bytes, only this
       // lane's own (frame byte 1 at j = 0, frame byte 0 at j = ND) is stored.
-      if (keep_in && j == 0) {
+      // Selects, not a branch, keep the loop one basic block.
+      if (j == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) out[r][0] = (out[r][0] & 0xFFFF00FFu) | ((R.own[r][0] & 0xFFu) << 8);
-      } else if (keep_in) {
+        for (int r = 0; r < 4; r++)
+          out[r][0] = keep_in ? (out[r][0] & 0xFFFF00FFu) | ((R.own[r][0] & 0xFFu) << 8) : out[r][0];
+      } else if (j == ND) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) out[r][ND] = (out[r][ND] & 0xFFFFFF00u) | (R.own[r][ND - 1] >> 24);
+        for (int r = 0; r < 4; r++)
+          out[r][ND] = keep_in ? (out[r][ND] & 0xFFFFFF00u) | (R.own[r][ND - 1] >> 24) : out[r][ND];
       }
     }
   }
