@@ -329,6 +329,10 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
 #pragma unroll
   for (int i = 0; i < NG; i++) wg[i] = fmix32((g0 + (u32)i) * 0x9E3779B1u + yzt);
   const u32 jb = (S >= 32) ? 0u : (bx0 & 15u);
+  // S < 32: the lane's S/2 + 1 <= 9 blocks start at block jb of group g0, so one
+  // 32-bit word (the two groups' words shifted by 2 * jb) holds all their
+  // variants at compile-time offsets.
+  const u32 vlane = (S < 32) ? (u32)(((((u64)wg[1]) << 32) | wg[0]) >> (2u * jb)) : 0u;
 
   // Reserved bytes read as STATIC: the frame becomes categories in one
   // (rarely taken) uniform branch, so that the evaluation below is one basic
@@ -358,11 +362,14 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
         e[1] = 0x0101010101010101ull;
         continue;
       }
-      const u32 kb = jb + (u32)(2 * j + h);
-      u32 w = wg[0];
-#pragma unroll
-      for (int i = 1; i < NG; i++) w = (kb >> 4) == (u32)i ? wg[i] : w;
-      const u32 v = (w >> (2u * (kb & 15u))) & 3u;
+      u32 v;
+      if constexpr (S < 32) {
+        v = (vlane >> (2 * (2 * j + h))) & 3u;
+      } else {  // jb = 0: the group and the offset are compile-time
+        constexpr int kb = 0;
+        const int k = kb + 2 * j + h;
+        v = (wg[k >> 4] >> (2 * (k & 15))) & 3u;
+      }
       const u32 s = h ? (states >> 16) : (states & 0xFFFFu);
       e[h] = rule[(v << 16) | s];
     }
