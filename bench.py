@@ -39,7 +39,7 @@ BYTES_PER_UPDATE = 2
 # command (profiles/tick_kernel_boxes512.md): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB
 # per dispatch (factors from a copy of known size), averaged over its skipping
 # and hashing ticks.
-PMC_TRAFFIC_BOXES512 = 170.3e6
+PMC_TRAFFIC_BOXES512 = 171.2e6
 METRIC = "Gvoxel-updates/s + %HBM roofline at 512^3/2048^3; batched vs plain; p99 tick ms"
 
 
