@@ -47,3 +47,28 @@ def test_pillars2048_fill_ticks_and_frame(gpu, O):
     assert (hit == whit).all() and (face == wface).all()
     assert np.abs(rgb - wrgb).max() <= 1e-5
     assert (hit != R.MISS).mean() > 0.05
+
+
+def test_pillars2048_skipping_matches_dense_and_fused(gpu):
+    # The default tick at this size packs the active segments of listed row
+    # quads; past the first two ticks most segments are settled.  Dense single
+    # ticks (checked against the oracle above) and fused pairs are the reference.
+    params = (256, 4, 32, 256, 1800, 256, 682, 256, 682, 32, 128)
+    dims = (2048, 2048, 2048)
+    announce("2048^3 pillars: 40 ticks, default (packed skipping) vs dense vs fused, hash every 10")
+    hashes = {}
+    for kind in ("skip", "dense", "fused"):
+        with gpu.World(dims, flags=0 if kind == "skip" else gpu.DENSE) as w:
+            if kind != "skip":
+                w.fuse = kind == "fused"
+                assert w.fuse == (kind == "fused")
+            else:
+                assert w.skip == gpu.SKIP_AUTO
+                w.sweep_stats(True)
+            w.fill_pillars(2, params)
+            hashes[kind] = [int(x) for x in w.run(40, 10)]
+            if kind == "skip":
+                st = w.sweep_stats(False)
+                assert st["violations"] == 0
+                assert st["swept"] < 0.5 * 40 * 2048 ** 3  # settled segments were skipped
+    assert hashes["skip"] == hashes["dense"] == hashes["fused"]
