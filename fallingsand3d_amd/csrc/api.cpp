@@ -261,6 +261,35 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
 
 }  // namespace
 
+namespace {
+
+// The cached graph of nticks ticks with a hash every hash_every from the current
+// buffer, captured and instantiated on first use.
+int world_graph(fs3d_world *w, int nticks, int hash_every, hipGraphExec_t *out) {
+  const auto key = std::make_tuple(nticks, hash_every, w->cur);
+  auto it = w->graphs.find(key);
+  if (it == w->graphs.end()) {
+    hipGraph_t graph = nullptr;
+    FS3D_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_ticks(w, nticks, hash_every, w->d_small, nullptr);
+    hipError_t e = hipStreamEndCapture(w->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    it = w->graphs.emplace(key, exec).first;
+  }
+  *out = it->second;
+  return FS3D_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *fs3d_last_error(void) { return g_error.c_str(); }
@@ -556,6 +585,18 @@ int fs3d_world_hash(fs3d_world *w, uint64_t *hash) {
   return clear_error();
 }
 
+int fs3d_world_prepare_run(fs3d_world *w, int32_t nticks, int32_t hash_every) {
+  if (int rc = check_world(w)) return rc;
+  if (nticks < 0 || hash_every < 0) return set_error(FS3D_ERR_INVALID, "nticks/hash_every < 0");
+  if (nticks == 0) return clear_error();
+  FS3D_HIP(hipSetDevice(w->dev));
+  const int nh = hash_every > 0 ? nticks / hash_every : 0;
+  if (int rc = ensure_small(w, nh + 1)) return rc;
+  hipGraphExec_t exec = nullptr;
+  if (int rc = world_graph(w, nticks, hash_every, &exec)) return rc;
+  return clear_error();
+}
+
 int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *hashes) {
   if (int rc = check_world(w)) return rc;
   if (nticks < 0 || hash_every < 0) return set_error(FS3D_ERR_INVALID, "nticks/hash_every < 0");
@@ -564,27 +605,11 @@ int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *
   if (nticks == 0) return clear_error();
   FS3D_HIP(hipSetDevice(w->dev));
   if (int rc = ensure_small(w, nh + 1)) return rc;
-  const auto key = std::make_tuple((int)nticks, (int)hash_every, w->cur);
-  auto it = w->graphs.find(key);
-  if (it == w->graphs.end()) {
-    hipGraph_t graph = nullptr;
-    FS3D_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_ticks(w, nticks, hash_every, w->d_small, nullptr);
-    hipError_t e = hipStreamEndCapture(w->stream, &graph);
-    if (rc) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-    it = w->graphs.emplace(key, exec).first;
-  }
+  hipGraphExec_t exec = nullptr;
+  if (int rc = world_graph(w, nticks, hash_every, &exec)) return rc;
   FS3D_HIP(launch_set_u64(w->d_small, w->tick, w->stream));
   FS3D_HIP(hipEventRecord(w->ev0, w->stream));
-  FS3D_HIP(hipGraphLaunch(it->second, w->stream));
+  FS3D_HIP(hipGraphLaunch(exec, w->stream));
   FS3D_HIP(hipEventRecord(w->ev1, w->stream));
   if (nh > 0)
     FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, (size_t)nh * sizeof(u64), hipMemcpyDeviceToHost, w->stream));

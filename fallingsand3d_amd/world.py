@@ -201,6 +201,11 @@ class World:
         check(lib().fs3d_world_run(self.handle, nticks, hash_every, ptr(out, c_u64)))
         return out[:nh]
 
+    def prepare_run(self, nticks, hash_every=0):
+        """Builds (captures and instantiates) the graph run(nticks, hash_every)
+        would launch from the current state, without running it."""
+        check(lib().fs3d_world_prepare_run(self.handle, nticks, hash_every))
+
     def last_run_ms_per_tick(self):
         ms = c_double(0)
         check(lib().fs3d_world_last_run_ms(self.handle, ctypes.byref(ms)))

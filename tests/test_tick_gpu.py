@@ -196,3 +196,19 @@ def test_cpp_wrapper_on_gpu(gpu, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     line = [ln for ln in r.stdout.split("\n") if ln.startswith("hash ")][0]
     assert int(line.split()[1]) == want[256]
+
+
+def test_prepare_run_builds_without_ticking(gpu):
+    announce("prepare_run: graph built ahead, state untouched, prepared run = plain run")
+    dims = (96, 70, 50)
+    with gpu.World(dims) as a, gpu.World(dims) as b:
+        for w in (a, b):
+            w.fill_rain(11, gpu.threshold(1, 3))
+        h0, t0 = a.hash(), a.tick_count
+        a.prepare_run(30, 5)
+        a.prepare_run(30, 5)  # cached: a second call is a no-op
+        assert a.hash() == h0 and a.tick_count == t0
+        assert [int(x) for x in a.run(30, 5)] == [int(x) for x in b.run(30, 5)]
+        assert (a.download() == b.download()).all()
+        with pytest.raises(gpu.Fs3dError):
+            a.prepare_run(-1, 0)
