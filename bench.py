@@ -19,7 +19,8 @@ Default run (one JSON line on stdout):
            same bytes, the practical bandwidth ceiling; batched: 4096 worlds against
            the one-launch-per-world loop; serving: the continuous-batching server
            under a Poisson load.
-  cpu_baseline  the oracle's tick on this host, one thread, a bounded sample.
+  cpu_baseline  the oracle's tick on this host, one thread, a bounded sample; its
+           "openmp" field times the same tick over the host threads the run may use.
 
 --only {boxes512,pillars2048,copy,batched,serving,render2048} runs one workload
 (for profiler runs) and prints its own JSON line.
@@ -260,21 +261,30 @@ def bench_serving(F, seconds=3.0, lam=2000.0, seed=5, round_ticks=8, check=8):
                 gvox_per_s=st["voxel_updates"] / wall / 1e9, busy_frac=st["busy_ms"] / (wall * 1e3))
 
 
-def cpu_baseline(seconds=12.0):
-    """The oracle's tick (table-driven, one thread) on the 512^3 box scene, timed
-    for about `seconds` of host work."""
+def cpu_baseline(seconds=12.0, omp_seconds=6.0):
+    """The oracle's tick (table-driven) on the 512^3 box scene: one thread for
+    about `seconds` of host work, then OpenMP over the host threads this run may
+    use (OMP_NUM_THREADS, else the CPU count) for about `omp_seconds`."""
     from oracle import oracle as O
-    g, _ = O.scene_boxes((512, 512, 512), 1)
-    O.set_threads(1)
-    t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < seconds:
-        g, _ = O.run(g, k, 1, 0, use_lut=True)
-        k += 1
-    dt = time.perf_counter() - t0
+
+    def timed(threads, budget):
+        g, _ = O.scene_boxes((512, 512, 512), 1)
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < budget:
+            g, _ = O.run(g, k, 1, 0, use_lut=True)
+            k += 1
+        return k, time.perf_counter() - t0
+
+    k, dt = timed(1, seconds)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    k2, dt2 = timed(threads, omp_seconds)
     O.set_threads(0)
     return dict(value=round(k * 512 ** 3 / dt / 1e9, 4), unit="Gvoxel-updates/s", cores=1, kind="port",
-                sample=f"{k} ticks of the 512^3 box scene (oracle table-driven tick, 1 thread, {dt:.1f} s)")
+                sample=f"{k} ticks of the 512^3 box scene (oracle table-driven tick, 1 thread, {dt:.1f} s)",
+                openmp=dict(value=round(k2 * 512 ** 3 / dt2 / 1e9, 4), cores=threads,
+                            sample=f"{k2} ticks of the same scene, OpenMP over {threads} threads, {dt2:.1f} s"))
 
 
 def main():
