@@ -97,11 +97,31 @@ class World {
     check(fs3d_world_run(w_, nticks, hash_every, h.empty() ? nullptr : h.data()));
     return h;
   }
+  void prepare_run(int32_t nticks, int32_t hash_every = 0) {
+    check(fs3d_world_prepare_run(w_, nticks, hash_every));
+  }
+  double last_run_ms_per_tick() const {
+    double ms = 0.0;
+    check(fs3d_world_last_run_ms(w_, &ms));
+    return ms;
+  }
   void sync() { check(fs3d_world_sync(w_)); }
 
   void save(const std::string &path) { check(fs3d_world_save(w_, path.c_str())); }
   void load(const std::string &path) { check(fs3d_world_load(w_, path.c_str())); }
   void set_skip(int32_t mode) { check(fs3d_world_set_skip(w_, mode)); }
+  int32_t skip() const {
+    int32_t m = 0;
+    check(fs3d_world_get_skip(w_, &m));
+    return m;
+  }
+  void set_fuse(bool on) { check(fs3d_world_set_fuse(w_, on ? 1 : 0)); }
+  bool fuse() const {
+    int32_t on = 0;
+    check(fs3d_world_get_fuse(w_, &on));
+    return on != 0;
+  }
+  void set_segment(int32_t seg_bytes) { check(fs3d_world_set_segment(w_, seg_bytes)); }
 
   void fill_rain(uint64_t seed, uint64_t thr) { check(fs3d_world_fill_rain(w_, seed, thr)); }
   int64_t fill_boxes(uint64_t seed, const fs3d_box_params &p) {
@@ -117,6 +137,12 @@ class World {
     hit.resize((size_t)W * H);
     face.resize((size_t)W * H);
     check(fs3d_world_render(w_, &cam, W, H, rgb.data(), hit.data(), face.data()));
+  }
+  // The voxel camera for the reference's key-driven camera state.
+  fs3d_camera key_camera(const float cam_pos[3], const float cam_rot[3], float aspect) const {
+    fs3d_camera cam;
+    check(fs3d_key_camera(dims_.data(), cam_pos, cam_rot, aspect, &cam));
+    return cam;
   }
 
  private:

@@ -196,6 +196,8 @@ def test_cpp_wrapper_on_gpu(gpu, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     line = [ln for ln in r.stdout.split("\n") if ln.startswith("hash ")][0]
     assert int(line.split()[1]) == want[256]
+    assert "skip 1 dense-skip 0 fuse 0/1 same 1 ms 1" in r.stdout
+    assert "camera forward 0.0 1.0 0.0" in r.stdout
 
 
 def test_prepare_run_builds_without_ticking(gpu):
