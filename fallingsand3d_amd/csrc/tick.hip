@@ -30,6 +30,7 @@ struct TickArgs {
   u64 *masks[2];        // per-parity change masks (SKIP only)
   u64 *stats;           // optional sweep counters (SKIP only): kHashParts slots + violations
   u32 *ctl;             // work-list counts, one per tick mod 3 (SKIP only)
+  int dilate;           // plan pass: odd ticks also list the neighbours of changed segments
 };
 
 __device__ __forceinline__ u64 tick_of(const TickArgs &a) { return a.tick_dev ? *a.tick_dev + a.k_off : a.t; }
@@ -164,8 +165,10 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
       u64 o = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) o |= in[k] ? v[k] : 0ull;
-      // the neighbours' bits matter on odd ticks only (straddling blocks)
-      act[i] = (own | o | (p ? (o << 1) | (o >> 1) : 0ull)) & lane_bits;
+      // The neighbours' bits matter on odd ticks only (straddling blocks), and
+      // only for the listed schedule: the packed one evaluates a straddling
+      // block next to an unlisted segment from its true bytes instead.
+      act[i] = (own | o | (p && a.dilate ? (o << 1) | (o >> 1) : 0ull)) & lane_bits;
     }
     bal[i] = __ballot(act[i] != 0);
     if (lane == 0) wcount[i * 4 + wv] = (u32)__builtin_popcountll(bal[i]);
@@ -263,12 +266,15 @@ __global__ __launch_bounds__(256) void tick_list_kernel(TickArgs a, const TickEn
 // search over the lanes' inclusive prefix of popcounts and the segment by a
 // select of the k-th set bit.  Items are in (entry, segment) order, so a lane's
 // left neighbour segment is active exactly when the entry's bit seg-1 is set,
-// and then it sits in the previous lane (or, for lane 0, in the previous group:
-// its edge bytes are read from memory).  On odd ticks the straddling block takes
-// the neighbour's edge bytes from the adjacent lane (DPP) or memory; next to a
-// settled segment it keeps its input, as in the other skipping schedules.  Change
-// bits are ORed per entry within a group (segmented scan) and routed to the
-// entry's lane, which writes the entry's mask word after the last group.
+// and then it sits in the previous lane (or, for lane 0, in the previous group).
+// On odd ticks a straddling block always sees its true bytes: the neighbour's
+// edge bytes come from the adjacent lane (DPP) when it holds that segment, and
+// from memory otherwise.  The plan pass therefore lists no neighbours of changed
+// segments (unlike the listed schedule), and a lane next to an unlisted segment
+// stores that segment's cell of the straddling block itself when it changes,
+// setting the segment's change bit.  Change bits are ORed per entry within a
+// group (segmented scan) and routed to the entry's lane, which writes the
+// entry's mask word after the last group.
 
 template <int S>
 __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEntry *__restrict__ list,
@@ -336,10 +342,16 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       R.zq = (int)zq;
       R.q = (long long)zq * nyq + yq;
       R.skip = false;
-      R.nl = active && seg > 0 && !left_act;
-      R.nr = active && seg < LR - 1 && !right_act;
-      R.ml = left_act && lane == 0;
-      R.mr = right_act && lane == 63;
+      // No keep-input: on odd ticks a straddling block always sees its true
+      // bytes, from the adjacent lane when it holds the neighbour segment and
+      // from memory otherwise; next to a segment that is not listed this lane
+      // also stores the neighbour's cell of that block.
+      R.nl = false;
+      R.nr = false;
+      R.ml = active && seg > 0 && (lane == 0 || !left_act);
+      R.mr = active && seg < LR - 1 && (lane == 63 || !right_act);
+      R.sl = active && seg > 0 && !left_act;
+      R.sr = active && seg < LR - 1 && !right_act;
       rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, true);
 #pragma unroll
       for (int r = 0; r < 4; r++) R.edge[r] = 0;
@@ -357,9 +369,13 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
         }
       }
       bool ch = false;
-      rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch);
-      // OR of the change bits of each entry's run of lanes in this group ...
+      u32 echg = 0;
+      rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch, &echg);
+      // OR of the change bits of each entry's run of lanes in this group (a
+      // neighbour's cell changed by this lane sets the neighbour's bit) ...
       u64 bits = (active && ch) ? (1ull << seg) : 0ull;
+      if (echg & 1u) bits |= 1ull << (seg - 1);
+      if (echg & 2u) bits |= 1ull << (seg + 1);
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const u64 ob = (u64)__shfl_up((unsigned long long)bits, d, 64);
@@ -513,6 +529,7 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
   a.masks[1] = masks ? masks[1] : nullptr;
   a.stats = masks ? stats : nullptr;
   a.ctl = masks ? ctl : nullptr;
+  a.dilate = 1;
   const bool hash = hash_parts != nullptr;
   const bool skip = masks != nullptr;
   switch (sh.seg) {
@@ -545,6 +562,7 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.masks[1] = masks[1];
   a.stats = stats;
   a.ctl = ctl;
+  a.dilate = 1;
   const size_t plan_blocks = (mask_words + 256 * kPlanPerThread - 1) / (256 * kPlanPerThread);
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)plan_blocks), dim3(256), 0, s, a, list, ctl);
   const dim3 grid(list_waves / 4);
@@ -579,6 +597,7 @@ hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.masks[1] = masks[1];
   a.stats = stats;
   a.ctl = ctl;
+  a.dilate = 0;
   const size_t plan_blocks = (mask_words + 256 * kPlanPerThread - 1) / (256 * kPlanPerThread);
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)plan_blocks), dim3(256), 0, s, a, list, ctl);
   const dim3 grid(pack_waves / 4);

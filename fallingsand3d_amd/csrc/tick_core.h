@@ -166,11 +166,12 @@ struct RowQuad {
   bool active;  // the lane holds a row quad of this tick
   bool skip;    // settled: nothing to evaluate or store
   bool loaded;  // own[] holds the grid bytes (false: skipped without loading)
-  // Segment-list tick only (rq_update's EDGES): the left / right neighbour
-  // segment is settled (nl / nr), or active but held by another wave, its edge
-  // byte then read from memory into edge[] (ml / mr; edge[r] = left byte << 24 |
-  // right byte).
-  bool nl, nr, ml, mr;
+  // Packed tick only (rq_update's EDGES): the left / right neighbour segment is
+  // settled and its straddling block keeps its input (nl / nr); or its edge
+  // bytes are read from memory into edge[] (ml / mr; edge[r] = left byte << 24 |
+  // right byte); sl / sr: the neighbour is not being updated, so this lane also
+  // stores the neighbour's cell of the straddling block when it changes.
+  bool nl, nr, ml, mr, sl, sr;
   u32 edge[4];
 };
 
@@ -262,19 +263,24 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // mr / edge instead of the adjacent lanes' load state.
 // P = t & 1 when known at compile time (even ticks need no neighbour exchange,
 // no shifted frame and no extra frame dword), or -1: the parity is read from t.
+// With EDGES, *edge_changed receives bit 0 / bit 1 when this lane stored the
+// left / right neighbour's cell (R.sl / R.sr) because it changed.
 template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P>
 __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
-                                           bool *changed) {
+                                           bool *changed, u32 *edge_changed) {
   constexpr int ND = S / 4;  // dwords per lane per row
   const int p = P < 0 ? (int)(t & 1u) : P;
   const int seg = R.seg;
 
   // ---- reserved bytes: wave-uniform slow path ----
+  // (edge bytes read from memory enter the frame too)
   u32 anyres = 0;
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int r = 0; r < 4; r++) {
 #pragma unroll
     for (int j = 0; j < ND; j++) anyres |= R.own[r][j] & 0xFCFCFCFCu;
+    if constexpr (EDGES) anyres |= R.edge[r] & 0xFC0000FCu;
+  }
   const bool slow = __any(anyres != 0);
 
   // ---- frame: even parity = own bytes; odd parity = [left byte, own, right byte] ----
@@ -400,6 +406,30 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
   if constexpr (SKIP || TRACK) {
     if (changed) *changed = R.loaded && dif != 0;
   }
+  // The neighbour's cell of a straddling block whose other lane is not updated
+  // this tick: stored here when it changed (categories 0..3; a reserved byte
+  // never moves).
+  if constexpr (EDGES) {
+    u32 echg = 0;
+    if (p) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (R.valid >> r & 1) {
+          const u32 old_l = R.edge[r] >> 24, new_l = out[r][0] & 0xFFu;
+          if (R.sl && old_l < 4u && new_l != old_l) {
+            dst[R.rowoff[r] - 1] = (u8)new_l;
+            echg |= 1u;
+          }
+          const u32 old_r = R.edge[r] & 0xFFu, new_r = (out[r][ND] >> 8) & 0xFFu;
+          if (R.sr && old_r < 4u && new_r != old_r) {
+            dst[R.rowoff[r] + S] = (u8)new_r;
+            echg |= 2u;
+          }
+        }
+      }
+    }
+    if (edge_changed) *edge_changed = echg;
+  }
 
   // ---- back to own bytes, restore reserved values, store, hash ----
   u64 acc = 0;
@@ -452,9 +482,9 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
 // one world per workgroup).
 template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false>
 __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
-                                         bool *changed = nullptr) {
-  if (t & 1u) return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 1>(R, dst, LR, t, rule, changed);
-  return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 0>(R, dst, LR, t, rule, changed);
+                                         bool *changed = nullptr, u32 *edge_changed = nullptr) {
+  if (t & 1u) return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 1>(R, dst, LR, t, rule, changed, edge_changed);
+  return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 0>(R, dst, LR, t, rule, changed, edge_changed);
 }
 
 // A single copy for both parities, for the batched kernels' persistent loops
@@ -462,7 +492,7 @@ __device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u
 template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP>
 __device__ __forceinline__ u64 rq_update_any(const RowQuad<S> &R, u8 *dst, int LR, u32 t,
                                              const u64 *__restrict__ rule, bool *changed = nullptr) {
-  return rq_update_p<S, HASH, SKIP, TRACK, false, -1>(R, dst, LR, t, rule, changed);
+  return rq_update_p<S, HASH, SKIP, TRACK, false, -1>(R, dst, LR, t, rule, changed, nullptr);
 }
 
 // Processes row-quad group `wave` of one tick at tick counter t (src may equal
