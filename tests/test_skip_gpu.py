@@ -180,3 +180,35 @@ def test_settled_world_sweeps_nothing(gpu, mode):
         assert 0 < st["swept"] < 40 * (128 * 64 * 32) // 50
         out = w.download()
         assert out[4, 30, 70] == 2 and int((out == 2).sum()) == int((g == 2).sum()) + 1
+
+
+@pytest.mark.parametrize("seg", [16, 32])
+@MODES
+def test_skip_reserved_bytes_on_segment_edges(gpu, O, seg, mode):
+    # Reserved bytes (which never move) on both sides of every segment edge,
+    # next to sand and fluid that keep moving: the straddling blocks read them
+    # through the adjacent lane or from memory (packed schedule), and a lane that
+    # stores a neighbour's cell must never overwrite one.
+    nx, ny, nz = 64 * seg, 10, 40
+    rng = np.random.default_rng(seg * 10 + mode)
+    g = np.zeros((nz, ny, nx), np.uint8)
+    g[0] = 1
+    edges = np.arange(seg, nx, seg)
+    for x in edges[::3]:
+        g[1:nz // 2, :, x - 1] = 200
+        g[1:nz // 2, ::2, x] = 77
+    g[nz // 2:] = np.where(rng.random((nz - nz // 2, ny, nx)) < 0.15, rng.choice([2, 3], size=(nz - nz // 2, ny, nx)), 0)
+    announce(f"reserved bytes on segment edges, nx={nx} seg={seg} mode={mode}")
+    with gpu.World((nx, ny, nz)) as w, gpu.World((nx, ny, nz), flags=gpu.DENSE) as d:
+        w.skip = mode
+        for x in (w, d):
+            x.segment = seg
+            x.upload(g)
+        w.sweep_stats(True)
+        assert [int(v) for v in w.run(60, 6)] == [int(v) for v in d.run(60, 6)]
+        out = w.download()
+        assert (out == d.download()).all()
+        want, _ = O.run(g, 0, 60, use_lut=True)
+        assert (out == want).all()
+        assert ((out == 200) == (g == 200)).all() and ((out == 77) == (g == 77)).all()
+        assert w.sweep_stats(False)["violations"] == 0
