@@ -16,7 +16,9 @@ struct RenderCam {
 };
 
 hipError_t launch_sdf(const float cam_pos[3], float aspect, int W, int H, float *d_rgba, hipStream_t s);
-// Brick occupancy words of a grid (8x8x8 bricks, 64 bricks per word along x).
+// u64 words of a grid's occupancy maps: the coarse map (one bit per 8x8x8
+// brick, 64 bricks per word along x) followed by the fine map (one word per
+// brick, one bit per 2x2x2 sub-brick).
 size_t occ_words(int nx, int ny, int nz);
 // Builds the occupancy map in d_occ (occ_words u64), then casts the rays.
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,

@@ -132,12 +132,18 @@ __device__ __forceinline__ int palette_index(u32 m) {
   return m == FS3D_STATIC ? 127 : m == FS3D_SAND ? 162 : m == FS3D_FLUID ? 137 : (int)m;
 }
 
-// Brick occupancy of a grid: one bit per 8x8x8 brick (set when any cell is not
-// EMPTY), occ[(bz * bny + by) * bwords + bx / 64] bit bx % 64.  One wave covers 64
-// bricks along x of one brick row: each lane ORs the 64 rows of 8 bytes of its
-// brick (loads coalesced across the wave), bytes past nx masked off.
+// Occupancy maps of a grid, two levels:
+//   coarse: one bit per 8x8x8 brick (set when any cell is not EMPTY),
+//           coarse[(bz * bny + by) * bwords + bx / 64] bit bx % 64;
+//   fine:   one u64 per brick, fine[(bz * bny + by) * bnx + bx], with bit
+//           ((x & 7) >> 1) | (((y & 7) >> 1) << 2) | (((z & 7) >> 1) << 4) set
+//           when the 2x2x2 sub-brick holding (x, y, z) has a non-EMPTY cell.
+// A coarse bit is set exactly when the brick's fine word is non-zero.  Cells
+// outside the grid count as EMPTY.  One wave covers 64 bricks along x of one
+// brick row: each lane ORs the 64 rows of 8 bytes of its brick (loads coalesced
+// across the wave, eight in flight per lane), bytes past nx masked off.
 __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, int bnx, int bny,
-                                                  int bnz, int bwords, u64 *occ) {
+                                                  int bnz, int bwords, u64 *coarse, u64 *fine) {
   const int lane = threadIdx.x & 63;
   const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long long nwords = (long long)bwords * bny * bnz;
@@ -150,29 +156,107 @@ __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int 
   if (bx < bnx) {
     const int x0 = bx * 8;
     const int keep = min(8, nx - x0);
-    const u64 mask = keep >= 8 ? ~0ull : ((1ull << (8 * keep)) - 1ull);
-    for (int dz = 0; dz < 8; dz++) {
-      const int z = bz * 8 + dz;
-      if (z >= nz) break;
-      u32x2 v[8];
+    const u32 mlo = keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
+    const u32 mhi = keep >= 8 ? ~0u : (keep <= 4 ? 0u : ((1u << (8 * (keep - 4))) - 1u));
+    for (int dz = 0; dz < 8; dz += 2) {
+      u32x2 v[2][8];
 #pragma unroll
-      for (int dy = 0; dy < 8; dy++) {  // eight loads in flight, rows past ny clamped
-        const int y = min(by * 8 + dy, ny - 1);
-        // unaligned 8-byte load; at most 7 bytes past the row end, inside the
-        // next row or the buffer's guard bytes, and masked off
-        v[dy] = *reinterpret_cast<const u32x2_u *>(g + ((size_t)z * ny + y) * nx + x0);
+      for (int k = 0; k < 2; k++) {
+        const int z = min(bz * 8 + dz + k, nz - 1);
+#pragma unroll
+        for (int dy = 0; dy < 8; dy++) {  // rows past ny or nz are read clamped, then dropped
+          const int y = min(by * 8 + dy, ny - 1);
+          // unaligned 8-byte load; at most 7 bytes past the row end, inside the
+          // next row or the buffer's guard bytes, and masked off
+          v[k][dy] = *reinterpret_cast<const u32x2_u *>(g + ((size_t)z * ny + y) * nx + x0);
+        }
       }
 #pragma unroll
-      for (int dy = 0; dy < 8; dy++) acc |= (((u64)v[dy].y << 32) | v[dy].x) & mask;
+      for (int sy = 0; sy < 4; sy++) {
+        u32 lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const bool in = bz * 8 + dz + k < nz && by * 8 + 2 * sy + j < ny;
+            lo |= in ? v[k][2 * sy + j].x : 0u;
+            hi |= in ? v[k][2 * sy + j].y : 0u;
+          }
+        }
+        lo &= mlo;
+        hi &= mhi;
+        const u32 b4 = ((lo & 0xFFFFu) != 0 ? 1u : 0u) | ((lo >> 16) != 0 ? 2u : 0u) |
+                       ((hi & 0xFFFFu) != 0 ? 4u : 0u) | ((hi >> 16) != 0 ? 8u : 0u);
+        acc |= (u64)b4 << (4 * sy + 16 * (dz >> 1));
+      }
     }
+    fine[((long long)bz * bny + by) * bnx + bx] = acc;
   }
   const u64 bits = __ballot(acc != 0);
-  if (lane == 0) occ[wave] = bits;
+  if (lane == 0) coarse[wave] = bits;
 }
 
+template <typename T>
+__device__ __forceinline__ T pick(bool cx, bool cy, T x, T y, T z) {  // values, not lvalues: no address selects
+  return cx ? x : (cy ? y : z);
+}
+
+// State of one ray's cell walk.  advance() takes one step: the axis with the
+// smallest tnext (ties x, then y, then z); false when the walk ends (a zero
+// step or leaving the grid), else t = tnext[a], idx[a] += step[a],
+// tnext[a] += tdelta[a], f = the face entered through, and crossed says whether
+// the new cell lies in another 8x8x8 brick.
+struct Walk {
+  int i0, i1, i2;
+  float tn0, tn1, tn2, t;
+  int f;
+  float td0, td1, td2;
+  int s0, s1, s2;
+  int nx, ny, nz;
+  int fc0, fc1, fc2;  // face entered by a step along each axis
+
+  // Branch-free: every field is written on every step by a select, so the
+  // lanes of a wave stay together and the state stays in registers.
+  __device__ __forceinline__ bool advance(bool &crossed) {
+    const bool cx = tn0 <= tn1 && tn0 <= tn2;
+    const bool cy = !cx && tn1 <= tn2;
+    const int sa = pick(cx, cy, s0, s1, s2);
+    const int ia = pick(cx, cy, i0, i1, i2);
+    const int na = ia + sa;
+    const bool ok = sa != 0 && (unsigned)na < (unsigned)pick(cx, cy, nx, ny, nz);
+    const bool mx = ok && cx, my = ok && cy, mz = ok && !cx && !cy;
+    t = ok ? pick(cx, cy, tn0, tn1, tn2) : t;
+    f = ok ? pick(cx, cy, fc0, fc1, fc2) : f;
+    i0 = mx ? na : i0;
+    i1 = my ? na : i1;
+    i2 = mz ? na : i2;
+    tn0 = mx ? tn0 + td0 : tn0;
+    tn1 = my ? tn1 + td1 : tn1;
+    tn2 = mz ? tn2 + td2 : tn2;
+    crossed = ((na ^ ia) & ~7) != 0;
+    return ok;
+  }
+};
+
+// One thread per pixel.  The walk is the cell-by-cell Amanatides & Woo
+// traversal: at each cell, test it, then take the axis with the smallest
+// tnext (ties x, then y, then z), t = tnext[a], idx[a] += step[a] (leaving the
+// grid or a zero step ends the walk), tnext[a] += tdelta[a].  It is organised
+// per brick: on entering a brick the coarse bit and, when set, the fine word
+// are read once; a cell is loaded only when its sub-brick bit is set, every
+// other cell is EMPTY by construction of the maps.  The stepping arithmetic is
+// the same sequence of float operations whatever the maps say, so t, the hit
+// cell and the face are those of the plain walk.
+//
+// Bounds: every step moves one axis by one cell in that axis's fixed direction,
+// so a walk ends after at most nx + ny + nz steps, and a brick is left after at
+// most 7 in-brick steps per axis.  The plain walk's cap of nx + ny + nz + 3
+// tested cells is therefore never reached, and the per-brick count below caps
+// the outer loop the same way.
 __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, RenderCam cam,
                                                   int W, int H, float *rgb, u64 *hit, u8 *face,
-                                                  const u64 *__restrict__ occ, int bny, int bwords) {
+                                                  const u64 *__restrict__ coarse, const u64 *__restrict__ fine, int bnx,
+                                                  int bny, int bwords) {
   const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= W || py >= H) return;
   const size_t pix = (size_t)py * W + px;
@@ -229,56 +313,67 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
         tnext[a] = __builtin_inff();
       }
     }
-    int f = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
-    float t = tmin;
-    const int maxit = nx + ny + nz + 3;
-    // Inside a brick whose occupancy bit is clear every cell is EMPTY: the walk
-    // steps through it exactly as cell by cell, without loading the cells.
-    long long cur_brick = -1;
-    bool brick_empty = false;
-    for (int it = 0; it < maxit; it++) {
-      const size_t li = ((size_t)idx[2] * ny + idx[1]) * nx + idx[0];
-      const long long bw = ((long long)(idx[2] >> 3) * bny + (idx[1] >> 3)) * bwords + (idx[0] >> 9);
-      const long long brick = bw * 64 + ((idx[0] >> 3) & 63);
-      if (brick != cur_brick) {
-        cur_brick = brick;
-        brick_empty = ((occ[bw] >> ((idx[0] >> 3) & 63)) & 1ull) == 0;
+    const int f0 = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
+    Walk wk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], tmin, f0, tdelta[0], tdelta[1], tdelta[2],
+            step[0], step[1], step[2], nx, ny, nz, step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5};
+    const int maxb = (nx + 7) / 8 + (ny + 7) / 8 + (nz + 7) / 8 + 3;
+    long long cw = -1;
+    u64 cword = 0;
+    bool found = false;
+    size_t hli = 0;
+    u32 hm = 0;
+    for (int nb = 0; nb < maxb; nb++) {
+      const int ux = wk.i0 >> 3, uy = wk.i1 >> 3, uz = wk.i2 >> 3;
+      const long long w = ((long long)uz * bny + uy) * bwords + (ux >> 6);
+      if (w != cw) {  // rays along x reuse one coarse word for 64 bricks
+        cw = w;
+        cword = coarse[w];
       }
-      const u32 m = brick_empty ? 0u : g[li];
-      if (m != 0) {
-        const int ci = palette_index(m);
-        F3 n{0.0f, 0.0f, 0.0f};
-        if (f == 6) {
-          n = F3{-d[0], -d[1], -d[2]};
-        } else {
-          const float s = (f & 1) ? 1.0f : -1.0f;
-          n.x = (f >> 1) == 0 ? s : 0.0f;
-          n.y = (f >> 1) == 1 ? s : 0.0f;
-          n.z = (f >> 1) == 2 ? s : 0.0f;
-        }
-        const float h0 = o[0] + t * d[0], h1 = o[1] + t * d[1], h2 = o[2] + t * d[2];
-        const F3 dl = nrm3(h0 - cam.light[0], h1 - cam.light[1], h2 - cam.light[2]);
-        const float diff = fmaxf(0.05f, dot3(n, dl));
-        outc[0] = __uint_as_float(kPaletteBits[4 * ci + 0]) * diff;
-        outc[1] = __uint_as_float(kPaletteBits[4 * ci + 1]) * diff;
-        outc[2] = __uint_as_float(kPaletteBits[4 * ci + 2]) * diff;
-        hout = (u64)li;
-        fout = (u8)f;
-        break;
+      const u64 fw = ((cword >> (ux & 63)) & 1ull) ? fine[((long long)uz * bny + uy) * bnx + ux] : 0ull;
+      bool crossed = false, go = true;
+      if (fw == 0) {
+        do {
+          go = wk.advance(crossed);
+        } while (go && !crossed);
+      } else {
+        do {
+          const int sb = ((wk.i0 >> 1) & 3) | (((wk.i1 >> 1) & 3) << 2) | (((wk.i2 >> 1) & 3) << 4);
+          if ((fw >> sb) & 1ull) {
+            const size_t li = ((size_t)wk.i2 * ny + wk.i1) * nx + wk.i0;
+            const u32 m = g[li];
+            if (m != 0) {
+              hm = m;
+              hli = li;
+              found = true;
+              break;
+            }
+          }
+          go = wk.advance(crossed);
+        } while (go && !crossed);
       }
-      int a;
-      if (tnext[0] <= tnext[1] && tnext[0] <= tnext[2])
-        a = 0;
-      else if (tnext[1] <= tnext[2])
-        a = 1;
-      else
-        a = 2;
-      if (step[a] == 0) break;
-      t = tnext[a];
-      idx[a] += step[a];
-      if (idx[a] < 0 || idx[a] >= ni[a]) break;
-      tnext[a] = tnext[a] + tdelta[a];
-      f = 2 * a + (step[a] > 0 ? 0 : 1);
+      if (found || !go) break;
+    }
+    if (found) {
+      const int ci = palette_index(hm);
+      const int f = wk.f;
+      const float t = wk.t;
+      F3 n{0.0f, 0.0f, 0.0f};
+      if (f == 6) {
+        n = F3{-d[0], -d[1], -d[2]};
+      } else {
+        const float s = (f & 1) ? 1.0f : -1.0f;
+        n.x = (f >> 1) == 0 ? s : 0.0f;
+        n.y = (f >> 1) == 1 ? s : 0.0f;
+        n.z = (f >> 1) == 2 ? s : 0.0f;
+      }
+      const float h0 = o[0] + t * d[0], h1 = o[1] + t * d[1], h2 = o[2] + t * d[2];
+      const F3 dl = nrm3(h0 - cam.light[0], h1 - cam.light[1], h2 - cam.light[2]);
+      const float diff = fmaxf(0.05f, dot3(n, dl));
+      outc[0] = __uint_as_float(kPaletteBits[4 * ci + 0]) * diff;
+      outc[1] = __uint_as_float(kPaletteBits[4 * ci + 1]) * diff;
+      outc[2] = __uint_as_float(kPaletteBits[4 * ci + 2]) * diff;
+      hout = (u64)hli;
+      fout = (u8)f;
     }
   }
   rgb[pix * 3 + 0] = outc[0];
@@ -302,20 +397,25 @@ hipError_t launch_box_march(const float *d_origin, const float *d_local, int n, 
   return hipGetLastError();
 }
 
-size_t occ_words(int nx, int ny, int nz) {
+static size_t coarse_words(int nx, int ny, int nz) {
   const size_t bwords = ((size_t)(nx + 7) / 8 + 63) / 64;
   return bwords * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
+}
+
+size_t occ_words(int nx, int ny, int nz) {
+  return coarse_words(nx, ny, nz) + (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
 }
 
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
                       u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s) {
   const int bnx = (nx + 7) / 8, bny = (ny + 7) / 8, bnz = (nz + 7) / 8, bwords = (bnx + 63) / 64;
   const long long waves = (long long)bwords * bny * bnz;
+  u64 *coarse = d_occ, *fine = d_occ + coarse_words(nx, ny, nz);
   hipLaunchKernelGGL(occ_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g, nx, ny, nz, bnx, bny, bnz,
-                     bwords, d_occ);
+                     bwords, coarse, fine);
   const dim3 grid((W + 15) / 16, (H + 15) / 16);
-  hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face, d_occ, bny,
-                     bwords);
+  hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face, coarse, fine,
+                     bnx, bny, bwords);
   return hipGetLastError();
 }
 
