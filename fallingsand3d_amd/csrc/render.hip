@@ -158,36 +158,37 @@ __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int 
     const int keep = min(8, nx - x0);
     const u32 mlo = keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
     const u32 mhi = keep >= 8 ? ~0u : (keep <= 4 ? 0u : ((1u << (8 * (keep - 4))) - 1u));
-    for (int dz = 0; dz < 8; dz += 2) {
-      u32x2 v[2][8];
+    const int ylim = min(8, ny - by * 8), zlim = min(8, nz - bz * 8);  // rows of the brick inside the grid
+    u32 al[4] = {0, 0, 0, 0}, ah[4] = {0, 0, 0, 0};  // ORed rows of each 2-row pair of the current 2-layer pair
+    for (int dz = 0; dz < zlim; dz++) {
+      u32x2 v[8];
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int z = min(bz * 8 + dz + k, nz - 1);
-#pragma unroll
-        for (int dy = 0; dy < 8; dy++) {  // rows past ny or nz are read clamped, then dropped
-          const int y = min(by * 8 + dy, ny - 1);
-          // unaligned 8-byte load; at most 7 bytes past the row end, inside the
-          // next row or the buffer's guard bytes, and masked off
-          v[k][dy] = *reinterpret_cast<const u32x2_u *>(g + ((size_t)z * ny + y) * nx + x0);
-        }
+      for (int dy = 0; dy < 8; dy++) {  // eight loads in flight; rows past ny are read clamped, then dropped
+        const int y = min(by * 8 + dy, ny - 1);
+        // unaligned 8-byte load; at most 7 bytes past the row end, inside the
+        // next row or the buffer's guard bytes, and masked off
+        v[dy] = *reinterpret_cast<const u32x2_u *>(g + ((size_t)(bz * 8 + dz) * ny + y) * nx + x0);
       }
 #pragma unroll
       for (int sy = 0; sy < 4; sy++) {
-        u32 lo = 0, hi = 0;
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            const bool in = bz * 8 + dz + k < nz && by * 8 + 2 * sy + j < ny;
-            lo |= in ? v[k][2 * sy + j].x : 0u;
-            hi |= in ? v[k][2 * sy + j].y : 0u;
-          }
+        if (2 * sy < ylim) {
+          al[sy] |= v[2 * sy].x;
+          ah[sy] |= v[2 * sy].y;
         }
-        lo &= mlo;
-        hi &= mhi;
-        const u32 b4 = ((lo & 0xFFFFu) != 0 ? 1u : 0u) | ((lo >> 16) != 0 ? 2u : 0u) |
-                       ((hi & 0xFFFFu) != 0 ? 4u : 0u) | ((hi >> 16) != 0 ? 8u : 0u);
-        acc |= (u64)b4 << (4 * sy + 16 * (dz >> 1));
+        if (2 * sy + 1 < ylim) {
+          al[sy] |= v[2 * sy + 1].x;
+          ah[sy] |= v[2 * sy + 1].y;
+        }
+      }
+      if ((dz & 1) || dz + 1 == zlim) {  // a 2-layer pair is complete: its 16 sub-brick bits
+#pragma unroll
+        for (int sy = 0; sy < 4; sy++) {
+          const u32 lo = al[sy] & mlo, hi = ah[sy] & mhi;
+          const u32 b4 = ((lo & 0xFFFFu) != 0 ? 1u : 0u) | ((lo >> 16) != 0 ? 2u : 0u) |
+                         ((hi & 0xFFFFu) != 0 ? 4u : 0u) | ((hi >> 16) != 0 ? 8u : 0u);
+          acc |= (u64)b4 << (4 * sy + 16 * (dz >> 1));
+          al[sy] = ah[sy] = 0;
+        }
       }
     }
     fine[((long long)bz * bny + by) * bnx + bx] = acc;
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int 
 }
 
 template <typename T>
-__device__ __forceinline__ T pick(bool cx, bool cy, T x, T y, T z) {  // values, not lvalues: no address selects
+__host__ __device__ __forceinline__ T pick(bool cx, bool cy, T x, T y, T z) {  // values, not lvalues: no address selects
   return cx ? x : (cy ? y : z);
 }
 
@@ -217,7 +218,7 @@ struct Walk {
 
   // Branch-free: every field is written on every step by a select, so the
   // lanes of a wave stay together and the state stays in registers.
-  __device__ __forceinline__ bool advance(bool &crossed) {
+  __host__ __device__ __forceinline__ bool advance(bool &crossed) {
     const bool cx = tn0 <= tn1 && tn0 <= tn2;
     const bool cy = !cx && tn1 <= tn2;
     const int sa = pick(cx, cy, s0, s1, s2);
@@ -235,6 +236,82 @@ struct Walk {
     tn2 = mz ? tn2 + td2 : tn2;
     crossed = ((na ^ ia) & ~7) != 0;
     return ok;
+  }
+
+  // True when no chain tnext, tnext + tdelta, ... can reach NaN: tnext is not
+  // NaN, and tdelta is finite on every moving axis (on a still axis both are
+  // +inf, and +inf + +inf stays +inf).
+  __host__ __device__ __forceinline__ bool no_nan() const {
+    return tn0 == tn0 && tn1 == tn1 && tn2 == tn2 && (s0 == 0 || td0 < __builtin_inff()) &&
+           (s1 == 0 || td1 < __builtin_inff()) && (s2 == 0 || td2 < __builtin_inff());
+  }
+
+  // advance() repeated until the step that leaves the current 8x8x8 brick, for a
+  // brick whose cells are all EMPTY, on a walk where no_nan() holds: false when
+  // the walk ends inside the brick or on that step.
+  // The steps along one axis happen at the values of that axis's own addition
+  // chain tnext, tnext + tdelta, ..., which never decreases, and advance()
+  // takes them in increasing (value, axis) order.  With k the axis's number of
+  // steps to leave the brick's part of the grid, its leaving step comes at
+  // T = tnext after k - 1 additions; the walk's first leaving step is the
+  // least (T, axis), and by then each other axis has taken the steps of its
+  // chain that come before it.  An axis with a zero step has T = tnext = +inf:
+  // when it is least, advance() would pick it and end the walk.  The additions
+  // are the same ones advance() makes, so t, tnext and the indices come out
+  // bit for bit as after the single steps.
+  __host__ __device__ __forceinline__ bool leave_brick() {
+    const int k0 = s0 > 0 ? min(i0 | 7, nx - 1) - i0 + 1 : (s0 < 0 ? (i0 & 7) + 1 : 1);
+    const int k1 = s1 > 0 ? min(i1 | 7, ny - 1) - i1 + 1 : (s1 < 0 ? (i1 & 7) + 1 : 1);
+    const int k2 = s2 > 0 ? min(i2 | 7, nz - 1) - i2 + 1 : (s2 < 0 ? (i2 & 7) + 1 : 1);
+    const float T0 = chain_at(tn0, td0, k0 - 1), T1 = chain_at(tn1, td1, k1 - 1), T2 = chain_at(tn2, td2, k2 - 1);
+    const bool ex = T0 <= T1 && T0 <= T2;
+    const bool ey = !ex && T1 <= T2;
+    const bool ez = !ex && !ey;
+    const int se = pick(ex, ey, s0, s1, s2);
+    const float Te = pick(ex, ey, T0, T1, T2);
+    const int ke = pick(ex, ey, k0, k1, k2);
+    const int ne = pick(ex, ey, i0, i1, i2) + (se > 0 ? ke : -ke);
+    if (se == 0 || (unsigned)ne >= (unsigned)pick(ex, ey, nx, ny, nz)) return false;
+    // the other axes: steps at chain values before (Te, e); an axis below e
+    // also takes the steps that tie with Te
+    int n0, n1, n2;
+    const float c0 = chain_before(tn0, td0, k0, Te, !ex, &n0);
+    const float c1 = chain_before(tn1, td1, k1, Te, ez, &n1);
+    const float c2 = chain_before(tn2, td2, k2, Te, false, &n2);
+    const float Tn = Te + pick(ex, ey, td0, td1, td2);
+    i0 = ex ? ne : i0 + n0 * s0;
+    i1 = ey ? ne : i1 + n1 * s1;
+    i2 = ez ? ne : i2 + n2 * s2;
+    tn0 = ex ? Tn : c0;
+    tn1 = ey ? Tn : c1;
+    tn2 = ez ? Tn : c2;
+    t = Te;
+    f = pick(ex, ey, fc0, fc1, fc2);
+    return true;
+  }
+
+  // c after m (0..7) additions of d
+  static __host__ __device__ __forceinline__ float chain_at(float c, float d, int m) {
+#pragma unroll
+    for (int j = 1; j < 8; j++) c = j <= m ? c + d : c;
+    return c;
+  }
+
+  // The chain c, c + d, ... over its first k values: *n = how many come before
+  // T (or equal it when tie), and the value after that many additions.
+  static __host__ __device__ __forceinline__ float chain_before(float c, float d, int k, float T, bool tie, int *n) {
+    float r = c;
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const bool before = j < k && (c < T || (tie && c == T));
+      const float c1 = c + d;
+      r = before ? c1 : r;
+      cnt = before ? j + 1 : cnt;
+      c = c1;
+    }
+    *n = cnt;
+    return r;
   }
 };
 
@@ -316,6 +393,7 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
     const int f0 = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
     Walk wk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], tmin, f0, tdelta[0], tdelta[1], tdelta[2],
             step[0], step[1], step[2], nx, ny, nz, step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5};
+    const bool jump = wk.no_nan();
     const int maxb = (nx + 7) / 8 + (ny + 7) / 8 + (nz + 7) / 8 + 3;
     long long cw = -1;
     u64 cword = 0;
@@ -331,7 +409,9 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
       }
       const u64 fw = ((cword >> (ux & 63)) & 1ull) ? fine[((long long)uz * bny + uy) * bnx + ux] : 0ull;
       bool crossed = false, go = true;
-      if (fw == 0) {
+      if (fw == 0 && jump) {
+        go = wk.leave_brick();
+      } else if (fw == 0) {  // rays whose chains could reach NaN: the single steps
         do {
           go = wk.advance(crossed);
         } while (go && !crossed);
