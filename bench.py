@@ -148,7 +148,7 @@ def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
     a W x H voxel render after each, from a seeded orbit camera."""
     from fallingsand3d_amd import render as R
     n = 2048
-    frames, tick_ms, render_ms = [], [], []
+    frames, tick_ms, render_ms, steps = [], [], [], 0
     with F.World((n, n, n)) as w:
         w.fill_pillars(seed)
         w.run(every, 0)  # capture the graph
@@ -163,11 +163,50 @@ def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
             rgb, hit, face = R.render(w, cam, W, H)
             render_ms.append(R.last_render_ms(w))
             frames.append(float((hit != R.MISS).mean()))
+            t1 = time.perf_counter()
+            steps += voxel_steps(cam, w.dims, hit, W, H)
+            t0 += time.perf_counter() - t1  # host-side counting is not part of the loop's wall time
         wall = time.perf_counter() - t0
         h = w.hash()
     return dict(wall=wall, tick_ms=sum(tick_ms) / len(tick_ms), render_ms=sum(render_ms) / len(render_ms),
                 render_ms_max=max(render_ms), hit_frac=sum(frames) / len(frames), frames=len(frames), hash=int(h),
-                vox=n ** 3, ticks=ticks, rays=W * H)
+                vox=n ** 3, ticks=ticks, rays=W * H, voxel_steps=steps / len(frames))
+
+
+def voxel_steps(cam, dims, hit, W, H):
+    """Cells a frame's rays walked through: per ray, the sum over axes of the
+    distance between its first cell and its last one (the hit cell, or the cell
+    where it leaves the grid), plus one; each axis only moves one way in a
+    walk.  Computed on the host in fp32 from the camera, so a ray's count can
+    be off by a cell where its entry or exit point lies on a cell face."""
+    import numpy as np
+    nx, ny, nz = dims
+    f32 = np.float32
+    px = np.arange(W, dtype=f32)[None, :]
+    py = np.arange(H, dtype=f32)[:, None]
+    ux = ((f32(W) - px) - f32(0.5)) / f32(W) * f32(2) - f32(1)
+    uy = (((py + f32(0.5)) / f32(H)) * f32(2) - f32(1)) * (f32(1) / f32(cam.aspect))
+    ux, uy = np.broadcast_to(ux, (H, W)), np.broadcast_to(uy, (H, W))
+    b = np.array(cam.basis[:], f32).reshape(3, 3)
+    d = ux[..., None] * b[0] + uy[..., None] * b[1] + b[2]
+    d /= np.sqrt((d * d).sum(-1, keepdims=True))
+    o = np.array(cam.origin[:], f32)
+    n = np.array([nx, ny, nz], f32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t1, t2 = (0 - o) / d, (n - o) / d
+    lo = np.where(d == 0, -np.inf, np.minimum(t1, t2)).max(-1)
+    hi = np.where(d == 0, np.inf, np.maximum(t1, t2)).min(-1)
+    tmin = np.maximum(lo, 0)
+    inside = (tmin <= hi) & ~((d == 0) & ((o < 0) | (o > n))).any(-1)
+    ni = np.array([nx, ny, nz])
+    start = np.clip(np.floor(o + tmin[..., None] * d), 0, ni - 1).astype(np.int64)
+    end = np.clip(np.floor(o + hi[..., None] * d), 0, ni - 1).astype(np.int64)
+    hit = np.asarray(hit).reshape(H, W)
+    h = hit != np.iinfo(np.uint64).max
+    li = hit[h].astype(np.int64)
+    end[h] = np.stack([li % nx, (li // nx) % ny, li // (nx * ny)], -1)
+    steps = np.abs(end - start).sum(-1) + 1
+    return int(steps[inside].sum())
 
 
 def bench_copy(F, nbytes, iters=20):
@@ -382,6 +421,8 @@ def main():
             "wall_s": round(pr["wall"], 3), "tick_ms": round(pr["tick_ms"], 4),
             "render_ms_mean": round(pr["render_ms"], 3), "render_ms_max": round(pr["render_ms_max"], 3),
             "rays_per_s": round(pr["rays"] / (pr["render_ms"] * 1e-3), 1), "hit_fraction": round(pr["hit_frac"], 4),
+            "voxel_steps_per_frame": round(pr["voxel_steps"]),
+            "voxel_steps_per_s": round(pr["voxel_steps"] / (pr["render_ms"] * 1e-3), 1),
             "final_hash": pr["hash"]}
         ps, pd = p["skip"], p["dense"]
         extra["pillars2048"] = {
