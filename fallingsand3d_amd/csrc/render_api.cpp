@@ -110,6 +110,8 @@ M4 inverse(const M4 &m) {
 
 float radians(float deg) { return deg * 0.01745329251994329576923690768489f; }
 
+constexpr int kMaxImageSide = 1 << 16;  // 16x16-pixel tiles: at most 4096 tiles per side
+
 int check_cam(const fs3d_camera *cam) {
   if (!cam) return set_error(FS3D_ERR_INVALID, "camera is NULL");
   if (!(cam->aspect > 0.0f)) return set_error(FS3D_ERR_INVALID, "camera aspect must be > 0");
@@ -240,7 +242,7 @@ int fs3d_key_camera(const int32_t dims[3], const float cam_pos[3], const float c
 }
 
 int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, float *rgba) {
-  if (!cam_pos || !rgba || W < 1 || H < 1 || !(aspect > 0.0f))
+  if (!cam_pos || !rgba || W < 1 || H < 1 || W > kMaxImageSide || H > kMaxImageSide || !(aspect > 0.0f))
     return set_error(FS3D_ERR_INVALID, "render_sdf: bad arguments");
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -286,7 +288,8 @@ int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t 
                       uint8_t *face) {
   if (!w) return set_error(FS3D_ERR_INVALID, "world handle is NULL");
   if (int rc = check_cam(cam)) return rc;
-  if (W < 1 || H < 1 || !rgb || !hit || !face) return set_error(FS3D_ERR_INVALID, "render: bad image arguments");
+  if (W < 1 || H < 1 || W > kMaxImageSide || H > kMaxImageSide || !rgb || !hit || !face)
+    return set_error(FS3D_ERR_INVALID, "render: bad image arguments (1 <= W, H <= %d)", kMaxImageSide);
   FS3D_HIP(hipSetDevice(w->dev));
   const size_t px = (size_t)W * H;
   if (px > w->render_px) {

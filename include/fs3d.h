@@ -228,7 +228,8 @@ typedef struct fs3d_camera {
   float aspect;
 } fs3d_camera;
 /* HIP port of shaders/fs_raymarch.frag (sphere tracer; origin = cam_pos as in
- * shaders/fs_raymarch.vert:33): rgba[H][W][4] float, written to host memory. */
+ * shaders/fs_raymarch.vert:33): rgba[H][W][4] float, written to host memory.
+ * Images are 1..65536 pixels per side here and in fs3d_world_render. */
 int fs3d_render_sdf(const float cam_pos[3], float aspect, int32_t W, int32_t H, float *rgba);
 /* The scene's active marcher, shaders/raymarch_sphere.{vert,frag}, at its stage
  * boundaries (the rasteriser between them stays the caller's).  Vertex stage:

@@ -246,6 +246,15 @@ def test_box_march_against_analytic_box(O):
     assert inside[0, 0] == 1.0
 
 
+def test_image_size_limits():
+    """Image sides above 65536 are refused before any device work."""
+    with pytest.raises(F.Fs3dError) as ei:
+        R.render_sdf(W=65537, H=1)
+    assert ei.value.code == 1  # FS3D_ERR_INVALID
+    with pytest.raises(F.Fs3dError):
+        R.render_sdf(W=1, H=0)
+
+
 # ---- the ray caster's brick jump, built for the host ----------------------------------
 
 def test_brick_jump_equals_single_steps(tmp_path):
