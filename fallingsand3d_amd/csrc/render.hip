@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
     Walk wk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], tmin, f0, tdelta[0], tdelta[1], tdelta[2],
             step[0], step[1], step[2], nx, ny, nz, step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5};
     const bool jump = wk.no_nan();
-    const int maxb = (nx + 7) / 8 + (ny + 7) / 8 + (nz + 7) / 8 + 3;
+    const int maxb = (nx >> 3) + (ny >> 3) + (nz >> 3) + 6;  // bricks per axis + 1 each, + 3
     long long cw = -1;
     u64 cword = 0;
     bool found = false;
@@ -477,18 +477,20 @@ hipError_t launch_box_march(const float *d_origin, const float *d_local, int n, 
   return hipGetLastError();
 }
 
+static size_t bricks(int n) { return ((size_t)n + 7) / 8; }  // 8-cell bricks along an axis of n cells
+
 static size_t coarse_words(int nx, int ny, int nz) {
-  const size_t bwords = ((size_t)(nx + 7) / 8 + 63) / 64;
-  return bwords * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
+  const size_t bwords = (bricks(nx) + 63) / 64;
+  return bwords * bricks(ny) * bricks(nz);
 }
 
 size_t occ_words(int nx, int ny, int nz) {
-  return coarse_words(nx, ny, nz) + (size_t)((nx + 7) / 8) * (size_t)((ny + 7) / 8) * (size_t)((nz + 7) / 8);
+  return coarse_words(nx, ny, nz) + bricks(nx) * bricks(ny) * bricks(nz);
 }
 
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
                       u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s) {
-  const int bnx = (nx + 7) / 8, bny = (ny + 7) / 8, bnz = (nz + 7) / 8, bwords = (bnx + 63) / 64;
+  const int bnx = (int)bricks(nx), bny = (int)bricks(ny), bnz = (int)bricks(nz), bwords = (bnx + 63) / 64;
   const long long waves = (long long)bwords * bny * bnz;
   u64 *coarse = d_occ, *fine = d_occ + coarse_words(nx, ny, nz);
   hipLaunchKernelGGL(occ_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g, nx, ny, nz, bnx, bny, bnz,

@@ -87,6 +87,12 @@ def test_dda_scenes_and_cameras(gpu, O, pal):
             above = R.camera([nx * 0.5, ny * 0.5, nz * 3.0], [[-1, 0, 0], [0, 1, 0], [0, 0, -1]],
                              [0, 0, -5 * nz], 1.0)
             _compare(R, O, w, above, pal, 64, 64, (name, "above"))
+            # rays with a subnormal direction component (1/d is +inf on a moving axis):
+            # the walk takes single steps across empty bricks instead of jumping
+            # screen x moves along world x only by 1e-39: every ray's x component is subnormal
+            tiny = [[1e-39, 0, 0.8], [0, 0, 0.6], [0, 1, 0]]
+            skew = R.camera([nx * 0.37 + 0.1, -0.5 * ny, nz * 0.41], tiny, [0, -3 * ny, 0], 1.0)
+            _compare(R, O, w, skew, pal, 48, 48, (name, "subnormal"))
     finally:
         for _, w in scenes:
             w.close()
