@@ -215,17 +215,31 @@ struct Walk {
   int s0, s1, s2;
   int nx, ny, nz;
   int fc0, fc1, fc2;  // face entered by a step along each axis
+  int fl0, fl1, fl2;  // 7 on an axis stepping up, else 0: (i ^ fl) & 7 = in-brick steps left
+  int hi0, hi1, hi2;  // last index on an axis stepping up, else a bound no index reaches
+
+  // The walk from its first cell, as dda_kernel sets it up.
+  static __host__ __device__ __forceinline__ Walk start(const int idx[3], const float tnext[3], float t, int f,
+                                                        const float tdelta[3], const int step[3], int nx, int ny,
+                                                        int nz) {
+    const int big = 1 << 30;
+    return Walk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], t, f, tdelta[0], tdelta[1], tdelta[2],
+                step[0], step[1], step[2], nx, ny, nz,
+                step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5,
+                step[0] > 0 ? 7 : 0, step[1] > 0 ? 7 : 0, step[2] > 0 ? 7 : 0,
+                step[0] > 0 ? nx - 1 : big, step[1] > 0 ? ny - 1 : big, step[2] > 0 ? nz - 1 : big};
+  }
 
   // Branch-free: every field is written on every step by a select, so the
   // lanes of a wave stay together and the state stays in registers.
   __host__ __device__ __forceinline__ bool advance(bool &crossed) {
-    const bool cx = tn0 <= tn1 && tn0 <= tn2;
-    const bool cy = !cx && tn1 <= tn2;
+    const bool cx = (tn0 <= tn1) & (tn0 <= tn2);
+    const bool cy = !cx & (tn1 <= tn2);
     const int sa = pick(cx, cy, s0, s1, s2);
     const int ia = pick(cx, cy, i0, i1, i2);
     const int na = ia + sa;
-    const bool ok = sa != 0 && (unsigned)na < (unsigned)pick(cx, cy, nx, ny, nz);
-    const bool mx = ok && cx, my = ok && cy, mz = ok && !cx && !cy;
+    const bool ok = (sa != 0) & ((unsigned)na < (unsigned)pick(cx, cy, nx, ny, nz));
+    const bool mx = ok & cx, my = ok & cy, mz = ok & !cx & !cy;
     t = ok ? pick(cx, cy, tn0, tn1, tn2) : t;
     f = ok ? pick(cx, cy, fc0, fc1, fc2) : f;
     i0 = mx ? na : i0;
@@ -260,18 +274,20 @@ struct Walk {
   // are the same ones advance() makes, so t, tnext and the indices come out
   // bit for bit as after the single steps.
   __host__ __device__ __forceinline__ bool leave_brick() {
-    const int k0 = s0 > 0 ? min(i0 | 7, nx - 1) - i0 + 1 : (s0 < 0 ? (i0 & 7) + 1 : 1);
-    const int k1 = s1 > 0 ? min(i1 | 7, ny - 1) - i1 + 1 : (s1 < 0 ? (i1 & 7) + 1 : 1);
-    const int k2 = s2 > 0 ? min(i2 | 7, nz - 1) - i2 + 1 : (s2 < 0 ? (i2 & 7) + 1 : 1);
+    // steps to leave the brick's part of the grid along each axis (on a still
+    // axis any value in 1..8: its chain is +inf throughout)
+    const int k0 = min((i0 ^ fl0) & 7, hi0 - i0) + 1;
+    const int k1 = min((i1 ^ fl1) & 7, hi1 - i1) + 1;
+    const int k2 = min((i2 ^ fl2) & 7, hi2 - i2) + 1;
     const float T0 = chain_at(tn0, td0, k0 - 1), T1 = chain_at(tn1, td1, k1 - 1), T2 = chain_at(tn2, td2, k2 - 1);
-    const bool ex = T0 <= T1 && T0 <= T2;
-    const bool ey = !ex && T1 <= T2;
-    const bool ez = !ex && !ey;
+    const bool ex = (T0 <= T1) & (T0 <= T2);
+    const bool ey = !ex & (T1 <= T2);
+    const bool ez = !ex & !ey;
     const int se = pick(ex, ey, s0, s1, s2);
     const float Te = pick(ex, ey, T0, T1, T2);
-    const int ke = pick(ex, ey, k0, k1, k2);
-    const int ne = pick(ex, ey, i0, i1, i2) + (se > 0 ? ke : -ke);
-    if (se == 0 || (unsigned)ne >= (unsigned)pick(ex, ey, nx, ny, nz)) return false;
+    const int ne = pick(ex, ey, i0, i1, i2) + pick(ex, ey, k0, k1, k2) * se;
+    // (the state is not used after the walk ends)
+    const bool ok = (se != 0) & ((unsigned)ne < (unsigned)pick(ex, ey, nx, ny, nz));
     // the other axes: steps at chain values before (Te, e); an axis below e
     // also takes the steps that tie with Te
     int n0, n1, n2;
@@ -287,7 +303,7 @@ struct Walk {
     tn2 = ez ? Tn : c2;
     t = Te;
     f = pick(ex, ey, fc0, fc1, fc2);
-    return true;
+    return ok;
   }
 
   // c after m (0..7) additions of d
@@ -304,7 +320,7 @@ struct Walk {
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const bool before = j < k && (c < T || (tie && c == T));
+      const bool before = (j < k) & ((c < T) | (tie & (c == T)));  // & and |: no branches
       const float c1 = c + d;
       r = before ? c1 : r;
       cnt = before ? j + 1 : cnt;
@@ -391,8 +407,7 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
       }
     }
     const int f0 = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
-    Walk wk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], tmin, f0, tdelta[0], tdelta[1], tdelta[2],
-            step[0], step[1], step[2], nx, ny, nz, step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5};
+    Walk wk = Walk::start(idx, tnext, tmin, f0, tdelta, step, nx, ny, nz);
     const bool jump = wk.no_nan();
     const int maxb = (nx >> 3) + (ny >> 3) + (nz >> 3) + 6;  // bricks per axis + 1 each, + 3
     long long cw = -1;

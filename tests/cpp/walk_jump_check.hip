@@ -67,8 +67,7 @@ bool start_walk(const float o[3], const float d[3], int nx, int ny, int nz, Walk
     }
   }
   const int f0 = enter >= 0 ? 2 * enter + (d[enter] > 0.0f ? 0 : 1) : 6;
-  *w = Walk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], tmin, f0, tdelta[0], tdelta[1], tdelta[2],
-            step[0], step[1], step[2], nx, ny, nz, step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5};
+  *w = Walk::start(idx, tnext, tmin, f0, tdelta, step, nx, ny, nz);
   return true;
 }
 
