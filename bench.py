@@ -303,7 +303,9 @@ def bench_serving(F, seconds=3.0, lam=2000.0, seed=5, round_ticks=8, check=8):
 def cpu_baseline(seconds=12.0, omp_seconds=6.0):
     """The oracle's tick (table-driven) on the 512^3 box scene: one thread for
     about `seconds` of host work, then OpenMP over the host threads this run may
-    use (OMP_NUM_THREADS, else the CPU count) for about `omp_seconds`."""
+    use (OMP_NUM_THREADS, else the CPU count) for about `omp_seconds`; then two
+    ticks of the 2048^3 pillar scene with OpenMP (sampled: every voxel is
+    updated, as the oracle does not skip)."""
     from oracle import oracle as O
 
     def timed(threads, budget):
@@ -319,11 +321,23 @@ def cpu_baseline(seconds=12.0, omp_seconds=6.0):
     k, dt = timed(1, seconds)
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     k2, dt2 = timed(threads, omp_seconds)
+    # 2048^3: two ticks (one of each parity) of the pillar scene, sampled
+    from fallingsand3d_amd.world import PILLARS_2048
+    O.set_threads(threads)
+    g = O.scene_pillars((2048, 2048, 2048), 2, PILLARS_2048)
+    t0 = time.perf_counter()
+    g, _ = O.run(g, 0, 2, 0, use_lut=True)
+    dt3 = time.perf_counter() - t0
+    del g
     O.set_threads(0)
     return dict(value=round(k * 512 ** 3 / dt / 1e9, 4), unit="Gvoxel-updates/s", cores=1, kind="port",
                 sample=f"{k} ticks of the 512^3 box scene (oracle table-driven tick, 1 thread, {dt:.1f} s)",
                 openmp=dict(value=round(k2 * 512 ** 3 / dt2 / 1e9, 4), cores=threads,
-                            sample=f"{k2} ticks of the same scene, OpenMP over {threads} threads, {dt2:.1f} s"))
+                            sample=f"{k2} ticks of the same scene, OpenMP over {threads} threads, {dt2:.1f} s"),
+                pillars2048_sampled=dict(ms_per_tick=round(dt3 / 2 * 1e3, 1),
+                                         value=round(2 * 2048 ** 3 / dt3 / 1e9, 4), cores=threads,
+                                         sample="2 ticks (one of each parity) of the 2048^3 pillar scene, OpenMP over "
+                                                f"{threads} threads, {dt3:.1f} s (sampled)"))
 
 
 def main():

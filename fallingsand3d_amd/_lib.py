@@ -130,6 +130,8 @@ SIGNATURES = {
     "fs3d_render_box_march": (c_int, [P(c_float), P(c_float), c_i32, P(c_float)]),
     "fs3d_world_render": (c_int, [vp, P(Camera), c_i32, c_i32, P(c_float), P(c_u64), P(c_u8)]),
     "fs3d_world_last_render_ms": (c_int, [vp, P(c_double)]),
+    "fs3d_world_set_render_track": (c_int, [vp, c_i32]),
+    "fs3d_world_get_render_track": (c_int, [vp, P(c_i32)]),
     "fs3d_orbit_camera": (c_int, [P(c_i32), c_double, c_double, c_double, c_float, P(Camera)]),
     "fs3d_key_camera": (c_int, [P(c_i32), P(c_float), P(c_float), c_float, P(Camera)]),
     "fs3d_camera_globaldata": (c_int, [P(c_float), P(c_float), P(c_float)]),

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fs3d_internal.h"
+#include "fs3d_render.h"
 
 namespace fs3d {
 
@@ -111,6 +112,7 @@ void release(fs3d_world *w) {
   if (w->d_hit) (void)hipFree(w->d_hit);
   if (w->d_face) (void)hipFree(w->d_face);
   if (w->d_occ) (void)hipFree(w->d_occ);
+  if (w->d_dirty) (void)hipFree(w->d_dirty);
   if (w->d_small) (void)hipFree(w->d_small);
   for (int i = 0; i < 2; i++)
     if (w->d_masks[i]) (void)hipFree(w->d_masks[i]);
@@ -129,6 +131,7 @@ constexpr size_t kStatWords = (size_t)kHashParts * kHashStride + 1;
 // Every change bit set: the next two ticks evaluate every segment, after which
 // the masks describe real changes again.
 int mark_all(fs3d_world *w) {
+  w->occ_valid = false;  // a host-side change: the renderer rebuilds its maps
   for (int i = 0; i < 2; i++) FS3D_HIP(launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream));
   // work-list counts start from zero after any host-side change (tick, shape)
   FS3D_HIP(hipMemsetAsync(w->d_ctl, 0, 4 * sizeof(u32), w->stream));
@@ -138,6 +141,7 @@ int mark_all(fs3d_world *w) {
 // Sets the change bits of every lane segment whose blocks contain cell (x,y,z),
 // at both parities (segments x/S - 1 .. x/S + 1 of the row quads holding it).
 int mark_cell(fs3d_world *w, int x, int y, int z) {
+  w->occ_valid = false;
   long long idx[2];
   for (int q = 0; q < 2; q++) {
     const long long nyq = (w->ny + q + 1) >> 1;
@@ -175,10 +179,7 @@ int size_list(fs3d_world *w) {
   return FS3D_OK;
 }
 
-void drop_graphs(fs3d_world *w) {
-  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
-  w->graphs.clear();
-}
+void drop_graphs(fs3d_world *w) { world_drop_graphs(w); }
 
 // The schedule of the non-hash ticks of a skipping world: one wave per row-quad
 // group for small grids (no plan pass), packed segments for large ones (2048^3
@@ -251,6 +252,10 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
                              hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
         if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
       }
+      // once the world has been rendered: the bricks this tick changed
+      if (masks && w->occ_track)
+        FS3D_HIP(launch_occ_dirty(masks, w->tick + (u64)k, tick_dev, (u32)k, w->nx, w->ny, w->nz, w->shape.seg,
+                                  w->d_dirty, w->stream));
     }
     src ^= 1;
     k++;
@@ -289,6 +294,11 @@ int world_graph(fs3d_world *w, int nticks, int hash_every, hipGraphExec_t *out) 
 }
 
 }  // namespace
+
+void world_drop_graphs(fs3d_world *w) {
+  for (auto &kv : w->graphs) (void)hipGraphExecDestroy(kv.second);
+  w->graphs.clear();
+}
 
 extern "C" {
 
@@ -554,6 +564,7 @@ int fs3d_world_step(fs3d_world *w, int32_t nticks) {
   FS3D_HIP(hipSetDevice(w->dev));
   int fin = w->cur;
   if (int rc = enqueue_ticks(w, nticks, 0, nullptr, &fin)) return rc;
+  note_ticks(w);
   w->cur = fin;
   w->tick += (u64)nticks;
   return clear_error();
@@ -565,6 +576,7 @@ int fs3d_world_tick(fs3d_world *w, uint64_t *hash) {
   FS3D_HIP(hipSetDevice(w->dev));
   int fin = w->cur;
   if (int rc = enqueue_ticks(w, 1, 1, nullptr, &fin)) return rc;
+  note_ticks(w);
   FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, sizeof(u64), hipMemcpyDeviceToHost, w->stream));
   FS3D_HIP(hipStreamSynchronize(w->stream));
   w->cur = fin;
@@ -610,6 +622,7 @@ int fs3d_world_run(fs3d_world *w, int32_t nticks, int32_t hash_every, uint64_t *
   FS3D_HIP(launch_set_u64(w->d_small, w->tick, w->stream));
   FS3D_HIP(hipEventRecord(w->ev0, w->stream));
   FS3D_HIP(hipGraphLaunch(exec, w->stream));
+  note_ticks(w);
   FS3D_HIP(hipEventRecord(w->ev1, w->stream));
   if (nh > 0)
     FS3D_HIP(hipMemcpyAsync(w->h_pinned, w->d_small + 1, (size_t)nh * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
@@ -723,6 +736,7 @@ int fs3d_world_fill_boxes(fs3d_world *w, uint64_t seed, const fs3d_box_params *p
     (void)hipHostFree(host);
     return set_error(FS3D_ERR_INVALID, "box parameters do not fit the grid");
   }
+  w->occ_valid = false;
   hipError_t e = hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);

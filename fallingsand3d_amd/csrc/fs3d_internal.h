@@ -167,7 +167,11 @@ struct fs3d_world {
   fs3d::u64 *d_hit = nullptr;
   fs3d::u8 *d_face = nullptr;
   size_t render_px = 0;
-  fs3d::u64 *d_occ = nullptr;  // brick occupancy map of the renderer
+  fs3d::u64 *d_occ = nullptr;    // occupancy maps of the renderer (coarse, then fine)
+  fs3d::u64 *d_dirty = nullptr;  // bricks changed since the maps were built (coarse-map layout)
+  int track_mode = FS3D_TRACK_AUTO;
+  bool occ_track = false;  // set by a render when tracking: skipping ticks record changed bricks in d_dirty
+  bool occ_valid = false;  // d_occ, updated at the bricks of d_dirty, describes the current grid
   double last_render_ms = 0.0;
   // settled-segment skipping: mode (FS3D_SKIP_*) and per-parity change masks
   // (one u64 per row quad)
@@ -183,3 +187,11 @@ struct fs3d_world {
   // cached graphs keyed by (nticks, hash_every, cur)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 };
+
+// Destroys the world's cached tick graphs (they are captured again on next use).
+void world_drop_graphs(fs3d_world *w);
+// Ticks that record no changed bricks (skipping off, or tracking off) leave
+// the renderer's occupancy maps stale.
+inline void note_ticks(fs3d_world *w) {
+  if (!w->skip || !w->occ_track) w->occ_valid = false;
+}

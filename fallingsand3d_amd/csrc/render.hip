@@ -142,18 +142,28 @@ __device__ __forceinline__ int palette_index(u32 m) {
 // outside the grid count as EMPTY.  One wave covers 64 bricks along x of one
 // brick row: each lane ORs the 64 rows of 8 bytes of its brick (loads coalesced
 // across the wave, eight in flight per lane), bytes past nx masked off.
+//
+// With update set, only the bricks whose bit is set in dirty (same layout as the
+// coarse map) are read again; the other bricks keep their fine words.  Either
+// way the kernel clears the dirty words it covers.
 __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, int bnx, int bny,
-                                                  int bnz, int bwords, u64 *coarse, u64 *fine) {
+                                                  int bnz, int bwords, u64 *coarse, u64 *fine, u64 *dirty,
+                                                  int update) {
   const int lane = threadIdx.x & 63;
   const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long long nwords = (long long)bwords * bny * bnz;
   if (wave >= nwords) return;
+  const u64 dw = update ? dirty[wave] : ~0ull;  // one word per wave
+  if (dw == 0) return;
   const int wx = (int)(wave % bwords);
   const long long rest = wave / bwords;
   const int by = (int)(rest % bny), bz = (int)(rest / bny);
   const int bx = wx * 64 + lane;
+  const long long fi = ((long long)bz * bny + by) * bnx + bx;
+  const bool redo = bx < bnx && ((dw >> lane) & 1ull);
   u64 acc = 0;
-  if (bx < bnx) {
+  if (bx < bnx && !redo) acc = fine[fi];
+  if (redo) {
     const int x0 = bx * 8;
     const int keep = min(8, nx - x0);
     const u32 mlo = keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
@@ -191,10 +201,63 @@ __global__ __launch_bounds__(256) void occ_kernel(const u8 *__restrict__ g, int 
         }
       }
     }
-    fine[((long long)bz * bny + by) * bnx + bx] = acc;
+    fine[fi] = acc;
   }
   const u64 bits = __ballot(acc != 0);
-  if (lane == 0) coarse[wave] = bits;
+  if (lane == 0) {
+    coarse[wave] = bits;
+    dirty[wave] = 0;
+  }
+}
+
+// After a skipping tick: marks in dirty (coarse-map layout) every brick that
+// holds a lane segment whose change bit the tick set.  The tick's masks are
+// those of its parity p: row quad (yq, zq) covers rows 2yq - p, 2yq - p + 1 and
+// layers 2zq - p, 2zq - p + 1, bit s covers x in [s * seg, (s + 1) * seg), and
+// every cell the tick changed lies in a segment whose bit it set.  One thread
+// per brick row (by, bz): it ORs the masks of the (up to 5 x 5) row quads that
+// share rows with it, then spreads each segment bit over the seg / 8 bricks of
+// that segment in the row's dirty words.
+__global__ __launch_bounds__(256) void occ_dirty_kernel(const u64 *__restrict__ m0, const u64 *__restrict__ m1,
+                                                        u64 t_host, const u64 *tick_dev, u32 k_off, int ny, int nz,
+                                                        int seg, int bnx, int bny, int bnz, int bwords,
+                                                        u64 *dirty) {
+  const long long row = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (row >= (long long)bny * bnz) return;
+  const int by = (int)(row % bny), bz = (int)(row / bny);
+  const u64 t = tick_dev ? *tick_dev + k_off : t_host;
+  const int p = (int)(t & 1u);
+  const u64 *m = p ? m1 : m0;
+  const int nyq = (ny + p + 1) >> 1, nzq = (nz + p + 1) >> 1;
+  const int yq0 = (by * 8 + p) >> 1, yq1 = min((min(by * 8 + 7, ny - 1) + p) >> 1, nyq - 1);
+  const int zq0 = (bz * 8 + p) >> 1, zq1 = min((min(bz * 8 + 7, nz - 1) + p) >> 1, nzq - 1);
+  // all 25 loads issued at once (indices clamped into the rows' range, the
+  // ones outside it dropped afterwards)
+  u64 v[5][5];
+#pragma unroll
+  for (int j = 0; j < 5; j++)
+#pragma unroll
+    for (int i = 0; i < 5; i++) v[j][i] = m[(long long)min(zq0 + j, zq1) * nyq + min(yq0 + i, yq1)];
+  u64 segs = 0;
+#pragma unroll
+  for (int j = 0; j < 5; j++)
+#pragma unroll
+    for (int i = 0; i < 5; i++) segs |= (zq0 + j <= zq1 && yq0 + i <= yq1) ? v[j][i] : 0ull;
+  if (segs == 0) return;
+  // segment s covers bricks [s << sh, (s + 1) << sh), sh = log2(seg / 8)
+  const int sh = __builtin_ctz((unsigned)seg) - 3;
+  const int per = 1 << sh, spw = 64 >> sh;  // bricks per segment, segments per dirty word
+  const u64 run = per == 64 ? ~0ull : ((1ull << per) - 1ull);
+  for (int wx = 0; wx < bwords; wx++) {
+    const u64 sw = spw == 64 ? segs : (segs >> (wx * spw)) & ((1ull << spw) - 1ull);
+    if (sw == 0) continue;
+    u64 bits = 0;
+    for (int j = 0; j < spw; j++)
+      if ((sw >> j) & 1ull) bits |= run << (j * per);
+    const int left = bnx - wx * 64;  // bricks of the row in this word
+    if (left < 64) bits &= (1ull << left) - 1ull;
+    if (bits) dirty[row * bwords + wx] |= bits;
+  }
 }
 
 template <typename T>
@@ -503,13 +566,24 @@ size_t occ_words(int nx, int ny, int nz) {
   return coarse_words(nx, ny, nz) + bricks(nx) * bricks(ny) * bricks(nz);
 }
 
+size_t dirty_words(int nx, int ny, int nz) { return coarse_words(nx, ny, nz); }
+
+hipError_t launch_occ_dirty(u64 *const masks[2], u64 t, const u64 *tick_dev, u32 k_off, int nx, int ny, int nz,
+                            int seg, u64 *d_dirty, hipStream_t s) {
+  const int bnx = (int)bricks(nx), bny = (int)bricks(ny), bnz = (int)bricks(nz), bwords = (bnx + 63) / 64;
+  const long long rows = (long long)bny * bnz;
+  hipLaunchKernelGGL(occ_dirty_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, masks[0], masks[1], t,
+                     tick_dev, k_off, ny, nz, seg, bnx, bny, bnz, bwords, d_dirty);
+  return hipGetLastError();
+}
+
 hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam, int W, int H, float *d_rgb,
-                      u64 *d_hit, u8 *d_face, u64 *d_occ, hipStream_t s) {
+                      u64 *d_hit, u8 *d_face, u64 *d_occ, u64 *d_dirty, bool update, hipStream_t s) {
   const int bnx = (int)bricks(nx), bny = (int)bricks(ny), bnz = (int)bricks(nz), bwords = (bnx + 63) / 64;
   const long long waves = (long long)bwords * bny * bnz;
   u64 *coarse = d_occ, *fine = d_occ + coarse_words(nx, ny, nz);
   hipLaunchKernelGGL(occ_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g, nx, ny, nz, bnx, bny, bnz,
-                     bwords, coarse, fine);
+                     bwords, coarse, fine, d_dirty, update ? 1 : 0);
   const dim3 grid((W + 15) / 16, (H + 15) / 16);
   hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face, coarse, fine,
                      bnx, bny, bwords);

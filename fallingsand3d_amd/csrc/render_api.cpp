@@ -112,6 +112,14 @@ float radians(float deg) { return deg * 0.01745329251994329576923690768489f; }
 
 constexpr int kMaxImageSide = 1 << 16;  // 16x16-pixel tiles: at most 4096 tiles per side
 
+// Grids from this size up track changed bricks between frames by default: a
+// full read of the grid costs more than about ten ticks of recording them.
+constexpr size_t kTrackMinVoxels = (size_t)1 << 29;
+
+bool track_wanted(const fs3d_world *w) {
+  return w->track_mode == FS3D_TRACK_ON || (w->track_mode == FS3D_TRACK_AUTO && w->n >= kTrackMinVoxels);
+}
+
 int check_cam(const fs3d_camera *cam) {
   if (!cam) return set_error(FS3D_ERR_INVALID, "camera is NULL");
   if (!(cam->aspect > 0.0f)) return set_error(FS3D_ERR_INVALID, "camera aspect must be > 0");
@@ -307,6 +315,18 @@ int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t 
     w->render_px = px;
   }
   if (!w->d_occ) FS3D_HIP(hipMalloc(&w->d_occ, occ_words(w->nx, w->ny, w->nz) * sizeof(u64)));
+  if (!w->d_dirty) {
+    const size_t bytes = dirty_words(w->nx, w->ny, w->nz) * sizeof(u64);
+    FS3D_HIP(hipMalloc(&w->d_dirty, bytes));
+    FS3D_HIP(hipMemsetAsync(w->d_dirty, 0, bytes, w->stream));
+  }
+  if (!w->occ_track && track_wanted(w)) {
+    // from now on skipping ticks record the bricks they change, so the next
+    // render reads only those; graphs captured before lack that step
+    FS3D_HIP(hipStreamSynchronize(w->stream));
+    world_drop_graphs(w);
+    w->occ_track = true;
+  }
   RenderCam rc_cam;
   std::memcpy(rc_cam.origin, cam->origin, sizeof rc_cam.origin);
   std::memcpy(rc_cam.basis, cam->basis, sizeof rc_cam.basis);
@@ -314,7 +334,8 @@ int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t 
   rc_cam.aspect = cam->aspect;
   FS3D_HIP(hipEventRecord(w->ev0, w->stream));
   FS3D_HIP(launch_dda(w->buf[w->cur], w->nx, w->ny, w->nz, rc_cam, W, H, w->d_rgb, w->d_hit, w->d_face, w->d_occ,
-                      w->stream));
+                      w->d_dirty, w->occ_valid, w->stream));
+  w->occ_valid = true;
   FS3D_HIP(hipEventRecord(w->ev1, w->stream));
   FS3D_HIP(hipMemcpyAsync(rgb, w->d_rgb, px * 3 * sizeof(float), hipMemcpyDeviceToHost, w->stream));
   FS3D_HIP(hipMemcpyAsync(hit, w->d_hit, px * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
@@ -323,6 +344,27 @@ int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t 
   float ms = 0.f;
   FS3D_HIP(hipEventElapsedTime(&ms, w->ev0, w->ev1));
   w->last_render_ms = ms;
+  return clear_error();
+}
+
+int fs3d_world_set_render_track(fs3d_world *w, int32_t mode) {
+  if (!w) return set_error(FS3D_ERR_INVALID, "world handle is NULL");
+  if (mode < FS3D_TRACK_AUTO || mode > FS3D_TRACK_OFF) return set_error(FS3D_ERR_INVALID, "track mode %d", mode);
+  w->track_mode = mode;
+  if (w->occ_track && !track_wanted(w)) {
+    // ticks stop recording changed bricks: the next render reads the whole grid
+    FS3D_HIP(hipSetDevice(w->dev));
+    FS3D_HIP(hipStreamSynchronize(w->stream));
+    world_drop_graphs(w);
+    w->occ_track = false;
+    w->occ_valid = false;
+  }
+  return clear_error();
+}
+
+int fs3d_world_get_render_track(const fs3d_world *w, int32_t *mode) {
+  if (!w || !mode) return set_error(FS3D_ERR_INVALID, "NULL argument");
+  *mode = w->track_mode;
   return clear_error();
 }
 

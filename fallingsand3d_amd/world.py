@@ -23,6 +23,7 @@ BOXES_512 = dict(side_min=8, side_max=64, pct=(20, 10, 5), max_boxes=1 << 20)
 PILLARS_2048 = (256, 4, 32, 256, 1800, 256, 682, 256, 682, 32, 128)
 DENSE = 1  # world flag: settled-segment skipping off
 SKIP_OFF, SKIP_AUTO, SKIP_LISTED, SKIP_WAVES, SKIP_SEGS = 0, 1, 2, 3, 4
+TRACK_AUTO, TRACK_ON, TRACK_OFF = 0, 1, 2  # renderer occupancy tracking between frames
 
 
 def device_count():
@@ -265,6 +266,19 @@ class World:
     @fuse.setter
     def fuse(self, on):
         check(lib().fs3d_world_set_fuse(self.handle, 1 if on else 0))
+
+    @property
+    def render_track(self):
+        """Occupancy tracking between frames: TRACK_AUTO (default), TRACK_ON or
+        TRACK_OFF.  With it, skipping ticks after a render record the 8^3 bricks
+        they change and the next render reads only those bricks of the grid."""
+        s = c_i32(0)
+        check(lib().fs3d_world_get_render_track(self.handle, ctypes.byref(s)))
+        return s.value
+
+    @render_track.setter
+    def render_track(self, mode):
+        check(lib().fs3d_world_set_render_track(self.handle, int(mode)))
 
     def sweep_stats(self, enable=True):
         """Counters since the previous call: voxels read and written, voxels only

@@ -249,6 +249,17 @@ int fs3d_render_box_march(const float *origin, const float *local, int32_t n, fl
 int fs3d_world_render(fs3d_world *w, const fs3d_camera *cam, int32_t W, int32_t H, float *rgb, uint64_t *hit,
                       uint8_t *face);
 int fs3d_world_last_render_ms(const fs3d_world *w, double *ms);
+/* Occupancy tracking between frames (no counterpart in the reference).  Each
+ * render reads the grid into occupancy maps before casting rays.  With tracking
+ * on, once a world has been rendered its skipping ticks record the 8^3 bricks
+ * they change (about 10 us per tick at 2048^3), and the next render reads only
+ * those bricks again; dense ticks and host-side edits make it read the whole
+ * grid.  Frames are identical either way.  FS3D_TRACK_AUTO (the default) tracks
+ * grids of at least 2^29 voxels, where a full read costs more than ten ticks of
+ * recording (2048^3: 1.55 ms against 0.56 ms per frame after 10 ticks). */
+enum { FS3D_TRACK_AUTO = 0, FS3D_TRACK_ON = 1, FS3D_TRACK_OFF = 2 };
+int fs3d_world_set_render_track(fs3d_world *w, int32_t mode);
+int fs3d_world_get_render_track(const fs3d_world *w, int32_t *mode);
 /* Seeded-orbit camera: eye at angle/elevation (radians) around the grid centre,
  * dist_scale * max(dims) away, looking at the centre with +z up on screen. */
 int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, double elev, float aspect,

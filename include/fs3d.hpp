@@ -116,6 +116,8 @@ class World {
     return m;
   }
   void set_fuse(bool on) { check(fs3d_world_set_fuse(w_, on ? 1 : 0)); }
+  // FS3D_TRACK_AUTO, FS3D_TRACK_ON or FS3D_TRACK_OFF: renderer occupancy tracking between frames
+  void set_render_track(int mode) { check(fs3d_world_set_render_track(w_, mode)); }
   bool fuse() const {
     int32_t on = 0;
     check(fs3d_world_get_fuse(w_, &on));

@@ -42,11 +42,20 @@ def test_pillars2048_fill_ticks_and_frame(gpu, O):
         announce("2048^3 pillars: one 1920x1080 frame vs oracle")
         cam = R.orbit_camera(w.dims, orbit_angles(2, 1)[0], 1.4, 0.35, 1920 / 1080)
         rgb, hit, face = R.render(w, cam, 1920, 1080)
-    ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
-    wrgb, whit, wface = O.render_dda(g, ocam, pal, 1920, 1080)
-    assert (hit == whit).all() and (face == wface).all()
-    assert np.abs(rgb - wrgb).max() <= 1e-5
-    assert (hit != R.MISS).mean() > 0.05
+        ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
+        wrgb, whit, wface = O.render_dda(g, ocam, pal, 1920, 1080)
+        assert (hit == whit).all() and (face == wface).all()
+        assert np.abs(rgb - wrgb).max() <= 1e-5
+        assert (hit != R.MISS).mean() > 0.05
+        announce("2048^3 pillars: four more (packed skipping) ticks, a frame from the changed bricks' maps")
+        w.step(4)
+        g, _ = O.run(g, 2, 4, use_lut=True)
+        assert w.hash() == O.state_hash(g)
+        rgb, hit, face = R.render(w, cam, 960, 540)
+        ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
+        wrgb, whit, wface = O.render_dda(g, ocam, pal, 960, 540)
+        assert (hit == whit).all() and (face == wface).all()
+        assert np.abs(rgb - wrgb).max() <= 1e-5
 
 
 def test_pillars2048_skipping_matches_dense_and_fused(gpu):
