@@ -342,28 +342,51 @@ struct Walk {
     const int k0 = min((i0 ^ fl0) & 7, hi0 - i0) + 1;
     const int k1 = min((i1 ^ fl1) & 7, hi1 - i1) + 1;
     const int k2 = min((i2 ^ fl2) & 7, hi2 - i2) + 1;
-    const float T0 = chain_at(tn0, td0, k0 - 1), T1 = chain_at(tn1, td1, k1 - 1), T2 = chain_at(tn2, td2, k2 - 1);
-    const bool ex = (T0 <= T1) & (T0 <= T2);
-    const bool ey = !ex & (T1 <= T2);
-    const bool ez = !ex & !ey;
+    // The leaving axis e and its leaving time Te.  First from estimates: on a
+    // moving axis E = tnext + (k - 1) * tdelta is within 10 * 2^-24 *
+    // max(|tnext|, |E|) of the chain value T (k - 1 <= 7 rounded additions
+    // against one rounded product and sum), so when the least E is below each
+    // other one by more than 2^-20 times the two axes' scales summed, the chain
+    // values order the same way with no tie, and only e's chain is needed.  A
+    // still axis (T = E = +inf) is apart from any finite E.  Near-ties take all
+    // three chains.
+    const float E0 = s0 ? tn0 + (float)(k0 - 1) * td0 : tn0;
+    const float E1 = s1 ? tn1 + (float)(k1 - 1) * td1 : tn1;
+    const float E2 = s2 ? tn2 + (float)(k2 - 1) * td2 : tn2;
+    bool ex = (E0 <= E1) & (E0 <= E2);
+    bool ey = !ex & (E1 <= E2);
+    bool ez = !ex & !ey;
+    const float Ee = pick(ex, ey, E0, E1, E2);
+    const float me = fmaxf(fabsf(pick(ex, ey, tn0, tn1, tn2)), fabsf(Ee));
+    const bool fin = Ee < 1e38f;
+    const bool a0 = ex | (s0 == 0 ? fin : (E0 - Ee > 0x1p-20f * (fmaxf(fabsf(tn0), fabsf(E0)) + me)));
+    const bool a1 = ey | (s1 == 0 ? fin : (E1 - Ee > 0x1p-20f * (fmaxf(fabsf(tn1), fabsf(E1)) + me)));
+    const bool a2 = ez | (s2 == 0 ? fin : (E2 - Ee > 0x1p-20f * (fmaxf(fabsf(tn2), fabsf(E2)) + me)));
+    float Te = chain_at(pick(ex, ey, tn0, tn1, tn2), pick(ex, ey, td0, td1, td2), pick(ex, ey, k0, k1, k2) - 1);
+    if (!(a0 & a1 & a2)) {
+      const float T0 = chain_at(tn0, td0, k0 - 1), T1 = chain_at(tn1, td1, k1 - 1), T2 = chain_at(tn2, td2, k2 - 1);
+      ex = (T0 <= T1) & (T0 <= T2);
+      ey = !ex & (T1 <= T2);
+      ez = !ex & !ey;
+      Te = pick(ex, ey, T0, T1, T2);
+    }
     const int se = pick(ex, ey, s0, s1, s2);
-    const float Te = pick(ex, ey, T0, T1, T2);
     const int ne = pick(ex, ey, i0, i1, i2) + pick(ex, ey, k0, k1, k2) * se;
     // (the state is not used after the walk ends)
     const bool ok = (se != 0) & ((unsigned)ne < (unsigned)pick(ex, ey, nx, ny, nz));
-    // the other axes: steps at chain values before (Te, e); an axis below e
-    // also takes the steps that tie with Te
-    int n0, n1, n2;
-    const float c0 = chain_before(tn0, td0, k0, Te, !ex, &n0);
-    const float c1 = chain_before(tn1, td1, k1, Te, ez, &n1);
-    const float c2 = chain_before(tn2, td2, k2, Te, false, &n2);
+    // The other two axes, a < b (e = 0: 1, 2; e = 1: 0, 2; e = 2: 0, 1): steps
+    // at chain values before (Te, e); an axis below e also takes the steps that
+    // tie with Te (a unless e = 0, b when e = 2).
+    int na, nb;
+    const float ca = chain_before(ex ? tn1 : tn0, ex ? td1 : td0, ex ? k1 : k0, Te, !ex, &na);
+    const float cb = chain_before(ez ? tn1 : tn2, ez ? td1 : td2, ez ? k1 : k2, Te, ez, &nb);
     const float Tn = Te + pick(ex, ey, td0, td1, td2);
-    i0 = ex ? ne : i0 + n0 * s0;
-    i1 = ey ? ne : i1 + n1 * s1;
-    i2 = ez ? ne : i2 + n2 * s2;
-    tn0 = ex ? Tn : c0;
-    tn1 = ey ? Tn : c1;
-    tn2 = ez ? Tn : c2;
+    i0 = ex ? ne : i0 + na * s0;
+    i1 = ey ? ne : i1 + (ex ? na : nb) * s1;
+    i2 = ez ? ne : i2 + nb * s2;
+    tn0 = ex ? Tn : ca;
+    tn1 = ey ? Tn : (ex ? ca : cb);
+    tn2 = ez ? Tn : cb;
     t = Te;
     f = pick(ex, ey, fc0, fc1, fc2);
     return ok;

@@ -80,13 +80,22 @@ int main(int argc, char **argv) {
   long long rays = 0, bricks = 0, ties = 0, bad = 0;
   for (long trial = 0; trial < trials; trial++) {
     const int n[3] = {1 + (int)(rng() % 70), 1 + (int)(rng() % 70), 1 + (int)(rng() % 70)};
-    const int mode = (int)(rng() % 6);
+    const int mode = (int)(rng() % 8);
     float o[3], dd[3], d[3];
     for (int a = 0; a < 3; a++)  // integer and half-integer origins, or anywhere around the grid
       o[a] = mode < 2 ? (float)(rng() % (n[a] + 1)) + (mode == 1 ? 0.5f : 0.0f) : (U(rng) * 1.6f - 0.3f) * n[a];
     for (int a = 0; a < 3; a++)  // axis-aligned and diagonal, small integer ratios, or any direction
       dd[a] = mode == 3 ? (float)((int)(rng() % 3) - 1)
                         : (mode == 4 ? (float)((int)(rng() % 7) - 3) : U(rng) * 2.0f - 1.0f);
+    if (mode >= 6) {  // near-ties: two or three components equal up to a few ulps
+      const float base = (U(rng) * 0.8f + 0.2f) * (rng() & 1 ? 1.0f : -1.0f);
+      for (int a = 0; a < (mode == 6 ? 2 : 3); a++)
+        dd[a] = std::nextafter(base, (rng() & 1) ? 2.0f : -2.0f) * ((rng() & 3) ? 1.0f : -1.0f);
+      for (int a = 0; a < 3; a++)
+        if (rng() & 1) dd[a] = std::nextafter(dd[a], 0.0f);
+      for (int a = 0; a < 3; a++)
+        if (mode == 7 || a < 2) o[a] = (float)(rng() % (n[a] + 1));
+    }
     const float l = std::sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
     if (!(l > 0.0f)) continue;
     for (int a = 0; a < 3; a++) d[a] = dd[a] / l;
