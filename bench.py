@@ -149,12 +149,15 @@ def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
     from fallingsand3d_amd import render as R
     n = 2048
     frames, tick_ms, render_ms, steps = [], [], [], 0
+    angles = orbit_angles(seed, ticks // every)
     with F.World((n, n, n)) as w:
         w.fill_pillars(seed)
-        w.run(every, 0)  # capture the graph
+        # a first render turns on occupancy tracking (this grid is above 2^29
+        # voxels), then the graph is captured with its changed-brick marking
+        R.render(w, R.orbit_camera(w.dims, angles[0], 1.4, 0.35, W / H), W, H)
+        w.run(every, 0)
         w.fill_pillars(seed)
         w.tick_count = 0
-        angles = orbit_angles(seed, ticks // every)
         t0 = time.perf_counter()
         for k, ang in enumerate(angles):
             w.run(every, 0)
