@@ -89,6 +89,17 @@ def last_render_ms(world):
     return ms.value
 
 
+def render_maps(world):
+    """(coarse, fine) occupancy maps of the world's last render, as uint64
+    arrays in the layouts fs3d_world_render_maps documents."""
+    nx, ny, nz = world.dims
+    bnx, bny, bnz = (nx + 7) // 8, (ny + 7) // 8, (nz + 7) // 8
+    coarse = np.zeros(((bnx + 63) // 64) * bny * bnz, np.uint64)
+    fine = np.zeros(bnx * bny * bnz, np.uint64)
+    check(lib().fs3d_world_render_maps(world.handle, ptr(coarse, c_u64), coarse.size, ptr(fine, c_u64), fine.size))
+    return coarse, fine
+
+
 def camera_globaldata(cam_pos, cam_rot):
     """The reference's 76-float GlobalData block for a camera state."""
     p = np.ascontiguousarray(cam_pos, np.float32)

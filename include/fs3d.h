@@ -260,6 +260,16 @@ int fs3d_world_last_render_ms(const fs3d_world *w, double *ms);
 enum { FS3D_TRACK_AUTO = 0, FS3D_TRACK_ON = 1, FS3D_TRACK_OFF = 2 };
 int fs3d_world_set_render_track(fs3d_world *w, int32_t mode);
 int fs3d_world_get_render_track(const fs3d_world *w, int32_t *mode);
+/* Diagnostics (no counterpart in the reference): the occupancy maps of the last
+ * render, which describe the grid as it was then.  With bnx = ceil(nx / 8) (and
+ * likewise bny, bnz) and bw = ceil(bnx / 64):
+ *   coarse[(bz * bny + by) * bw + bx / 64] bit bx % 64: brick (bx, by, bz) holds
+ *     a non-EMPTY cell;
+ *   fine[(bz * bny + by) * bnx + bx] bit ((x & 7) >> 1) | (((y & 7) >> 1) << 2) |
+ *     (((z & 7) >> 1) << 4): the 2x2x2 sub-brick holding (x, y, z) does.
+ * n_coarse and n_fine are the capacities in words (at least bw * bny * bnz and
+ * bnx * bny * bnz).  FS3D_ERR_STATE before the first render.  Synchronises. */
+int fs3d_world_render_maps(fs3d_world *w, uint64_t *coarse, size_t n_coarse, uint64_t *fine, size_t n_fine);
 /* Seeded-orbit camera: eye at angle/elevation (radians) around the grid centre,
  * dist_scale * max(dims) away, looking at the centre with +z up on screen. */
 int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, double elev, float aspect,

@@ -22,10 +22,36 @@ def pal():
     return np.array(json.load(open(gold))["rgba"], np.float32)
 
 
+def expected_maps(g):
+    """The occupancy maps of grid g[z][y][x] (numpy, from their definition):
+    coarse bits per 8^3 brick and fine words with a bit per 2x2x2 sub-brick."""
+    nz, ny, nx = g.shape
+    bnx, bny, bnz = -(-nx // 8), -(-ny // 8), -(-nz // 8)
+    pad = np.zeros((bnz * 8, bny * 8, bnx * 8), bool)
+    pad[:nz, :ny, :nx] = g != 0
+    sub = pad.reshape(bnz * 4, 2, bny * 4, 2, bnx * 4, 2).any(axis=(1, 3, 5))  # [sz][sy][sx]
+    sub = sub.reshape(bnz, 4, bny, 4, bnx, 4).transpose(0, 2, 4, 1, 3, 5)     # [bz][by][bx][z][y][x]
+    weights = (np.uint64(1) << np.arange(64, dtype=np.uint64)).reshape(4, 4, 4)  # bit x + 4 y + 16 z
+    fine = (sub.astype(np.uint64) * weights).sum(axis=(3, 4, 5), dtype=np.uint64).reshape(-1)
+    bw = -(-bnx // 64)
+    occ = (fine != 0).reshape(bnz * bny, bnx)
+    coarse = np.zeros(bnz * bny * bw, np.uint64)
+    for bx in range(bnx):
+        coarse[np.arange(bnz * bny) * bw + bx // 64] |= occ[:, bx].astype(np.uint64) << np.uint64(bx % 64)
+    return coarse, fine
+
+
 def _frame(R, O, w, cam, pal, W, H, label):
     rgb, hit, face = R.render(w, cam, W, H)
+    g = w.download()
+    # the maps after a render describe the current grid exactly, however they
+    # were brought up to date
+    coarse, fine = R.render_maps(w)
+    want_c, want_f = expected_maps(g)
+    assert (fine == want_f).all(), (label, "fine", int((fine != want_f).sum()))
+    assert (coarse == want_c).all(), (label, "coarse")
     ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
-    wrgb, whit, wface = O.render_dda(w.download(), ocam, pal, W, H)
+    wrgb, whit, wface = O.render_dda(g, ocam, pal, W, H)
     assert (hit == whit).all(), (label, int((hit != whit).sum()))
     assert (face == wface).all(), label
     assert np.abs(rgb - wrgb).max() <= RGB_TOL, label
