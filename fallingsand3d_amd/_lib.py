@@ -132,7 +132,8 @@ SIGNATURES = {
     "fs3d_world_last_render_ms": (c_int, [vp, P(c_double)]),
     "fs3d_world_set_render_track": (c_int, [vp, c_i32]),
     "fs3d_world_get_render_track": (c_int, [vp, P(c_i32)]),
-    "fs3d_world_render_maps": (c_int, [vp, P(c_u64), ctypes.c_size_t, P(c_u64), ctypes.c_size_t]),
+    "fs3d_world_render_maps": (c_int, [vp, P(c_u64), ctypes.c_size_t, P(c_u64), ctypes.c_size_t, P(c_u64),
+                                       ctypes.c_size_t]),
     "fs3d_orbit_camera": (c_int, [P(c_i32), c_double, c_double, c_double, c_float, P(Camera)]),
     "fs3d_key_camera": (c_int, [P(c_i32), P(c_float), P(c_float), c_float, P(Camera)]),
     "fs3d_camera_globaldata": (c_int, [P(c_float), P(c_float), P(c_float)]),

@@ -260,6 +260,37 @@ __global__ __launch_bounds__(256) void occ_dirty_kernel(const u64 *__restrict__ 
   }
 }
 
+// The region map: one bit per 32^3 region (4 x 4 x 4 bricks), set when any of
+// its bricks' coarse bits is, region[(rz * rny + ry) * rwords + rx / 64] bit
+// rx % 64.  One wave per 64 regions along x; each lane ORs the 4-bit groups of
+// its 16 brick rows.
+__global__ __launch_bounds__(256) void region_kernel(const u64 *__restrict__ coarse, int bny, int bnz, int bwords,
+                                                     int rnx, int rny, int rnz, int rwords, u64 *region) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= (long long)rwords * rny * rnz) return;
+  const int wx = (int)(wave % rwords);
+  const long long rest = wave / rwords;
+  const int ry = (int)(rest % rny), rz = (int)(rest / rny);
+  const int rx = wx * 64 + lane;
+  u64 any = 0;
+  if (rx < rnx) {
+    const int bx = rx * 4;  // the group's 4 bits never straddle a word: bx % 64 is a multiple of 4
+    for (int dz = 0; dz < 4; dz++) {
+      const int bz = rz * 4 + dz;
+      if (bz >= bnz) break;
+#pragma unroll
+      for (int dy = 0; dy < 4; dy++) {
+        const int by = ry * 4 + dy;
+        const u64 w = by < bny ? coarse[((long long)bz * bny + by) * bwords + (bx >> 6)] : 0ull;
+        any |= (w >> (bx & 63)) & 0xFull;
+      }
+    }
+  }
+  const u64 bits = __ballot(any != 0);
+  if (lane == 0) region[wave] = bits;
+}
+
 template <typename T>
 __host__ __device__ __forceinline__ T pick(bool cx, bool cy, T x, T y, T z) {  // values, not lvalues: no address selects
   return cx ? x : (cy ? y : z);
@@ -278,8 +309,9 @@ struct Walk {
   int s0, s1, s2;
   int nx, ny, nz;
   int fc0, fc1, fc2;  // face entered by a step along each axis
-  int fl0, fl1, fl2;  // 7 on an axis stepping up, else 0: (i ^ fl) & 7 = in-brick steps left
+  int up0, up1, up2;  // -1 on an axis stepping up, else 0: (i ^ (up & (L - 1))) & (L - 1) = steps left in an L-box
   int hi0, hi1, hi2;  // last index on an axis stepping up, else a bound no index reaches
+  float rtd0, rtd1, rtd2;  // 1 / tdelta on a moving axis (estimates only), else 0
 
   // The walk from its first cell, as dda_kernel sets it up.
   static __host__ __device__ __forceinline__ Walk start(const int idx[3], const float tnext[3], float t, int f,
@@ -289,8 +321,10 @@ struct Walk {
     return Walk{idx[0], idx[1], idx[2], tnext[0], tnext[1], tnext[2], t, f, tdelta[0], tdelta[1], tdelta[2],
                 step[0], step[1], step[2], nx, ny, nz,
                 step[0] > 0 ? 0 : 1, step[1] > 0 ? 2 : 3, step[2] > 0 ? 4 : 5,
-                step[0] > 0 ? 7 : 0, step[1] > 0 ? 7 : 0, step[2] > 0 ? 7 : 0,
-                step[0] > 0 ? nx - 1 : big, step[1] > 0 ? ny - 1 : big, step[2] > 0 ? nz - 1 : big};
+                step[0] > 0 ? -1 : 0, step[1] > 0 ? -1 : 0, step[2] > 0 ? -1 : 0,
+                step[0] > 0 ? nx - 1 : big, step[1] > 0 ? ny - 1 : big, step[2] > 0 ? nz - 1 : big,
+                step[0] ? 1.0f / tdelta[0] : 0.0f, step[1] ? 1.0f / tdelta[1] : 0.0f,
+                step[2] ? 1.0f / tdelta[2] : 0.0f};
   }
 
   // Branch-free: every field is written on every step by a select, so the
@@ -323,33 +357,37 @@ struct Walk {
            (s1 == 0 || td1 < __builtin_inff()) && (s2 == 0 || td2 < __builtin_inff());
   }
 
-  // advance() repeated until the step that leaves the current 8x8x8 brick, for a
-  // brick whose cells are all EMPTY, on a walk where no_nan() holds: false when
-  // the walk ends inside the brick or on that step.
+  // advance() repeated until the step that leaves the current L x L x L box
+  // (an 8^3 brick or a 32^3 region, aligned to multiples of L), for a box whose
+  // cells are all EMPTY, on a walk where no_nan() holds: false when the walk
+  // ends inside the box or on that step.
   // The steps along one axis happen at the values of that axis's own addition
   // chain tnext, tnext + tdelta, ..., which never decreases, and advance()
   // takes them in increasing (value, axis) order.  With k the axis's number of
-  // steps to leave the brick's part of the grid, its leaving step comes at
+  // steps to leave the box's part of the grid, its leaving step comes at
   // T = tnext after k - 1 additions; the walk's first leaving step is the
   // least (T, axis), and by then each other axis has taken the steps of its
   // chain that come before it.  An axis with a zero step has T = tnext = +inf:
   // when it is least, advance() would pick it and end the walk.  The additions
   // are the same ones advance() makes, so t, tnext and the indices come out
   // bit for bit as after the single steps.
-  __host__ __device__ __forceinline__ bool leave_brick() {
-    // steps to leave the brick's part of the grid along each axis (on a still
-    // axis any value in 1..8: its chain is +inf throughout)
-    const int k0 = min((i0 ^ fl0) & 7, hi0 - i0) + 1;
-    const int k1 = min((i1 ^ fl1) & 7, hi1 - i1) + 1;
-    const int k2 = min((i2 ^ fl2) & 7, hi2 - i2) + 1;
+  template <int L>
+  __host__ __device__ __forceinline__ bool leave_box() {
+    // steps to leave the box's part of the grid along each axis (on a still
+    // axis any value in 1..L: its chain is +inf throughout)
+    const int k0 = min((i0 ^ (up0 & (L - 1))) & (L - 1), hi0 - i0) + 1;
+    const int k1 = min((i1 ^ (up1 & (L - 1))) & (L - 1), hi1 - i1) + 1;
+    const int k2 = min((i2 ^ (up2 & (L - 1))) & (L - 1), hi2 - i2) + 1;
     // The leaving axis e and its leaving time Te.  First from estimates: on a
-    // moving axis E = tnext + (k - 1) * tdelta is within 10 * 2^-24 *
-    // max(|tnext|, |E|) of the chain value T (k - 1 <= 7 rounded additions
+    // moving axis E = tnext + (k - 1) * tdelta is within (L + 2) * 2^-24 *
+    // max(|tnext|, |E|) of the chain value T (k - 1 <= L - 1 rounded additions
     // against one rounded product and sum), so when the least E is below each
-    // other one by more than 2^-20 times the two axes' scales summed, the chain
-    // values order the same way with no tie, and only e's chain is needed.  A
-    // still axis (T = E = +inf) is apart from any finite E.  Near-ties take all
-    // three chains.
+    // other one by more than `margin` (2^-20 for L = 8, 2^-18 for L = 32) times
+    // the two axes' scales summed, the chain values order the same way with no
+    // tie, and only e's chain is needed.  A still axis (T = E = +inf) is apart
+    // from any finite E.  Near-ties take all three chains.
+    constexpr float margin = L <= 8 ? 0x1p-20f : 0x1p-18f;
+    static_assert(L <= 32, "the margin covers at most 31 additions");
     const float E0 = s0 ? tn0 + (float)(k0 - 1) * td0 : tn0;
     const float E1 = s1 ? tn1 + (float)(k1 - 1) * td1 : tn1;
     const float E2 = s2 ? tn2 + (float)(k2 - 1) * td2 : tn2;
@@ -359,12 +397,14 @@ struct Walk {
     const float Ee = pick(ex, ey, E0, E1, E2);
     const float me = fmaxf(fabsf(pick(ex, ey, tn0, tn1, tn2)), fabsf(Ee));
     const bool fin = Ee < 1e38f;
-    const bool a0 = ex | (s0 == 0 ? fin : (E0 - Ee > 0x1p-20f * (fmaxf(fabsf(tn0), fabsf(E0)) + me)));
-    const bool a1 = ey | (s1 == 0 ? fin : (E1 - Ee > 0x1p-20f * (fmaxf(fabsf(tn1), fabsf(E1)) + me)));
-    const bool a2 = ez | (s2 == 0 ? fin : (E2 - Ee > 0x1p-20f * (fmaxf(fabsf(tn2), fabsf(E2)) + me)));
-    float Te = chain_at(pick(ex, ey, tn0, tn1, tn2), pick(ex, ey, td0, td1, td2), pick(ex, ey, k0, k1, k2) - 1);
+    const bool a0 = ex | (s0 == 0 ? fin : (E0 - Ee > margin * (fmaxf(fabsf(tn0), fabsf(E0)) + me)));
+    const bool a1 = ey | (s1 == 0 ? fin : (E1 - Ee > margin * (fmaxf(fabsf(tn1), fabsf(E1)) + me)));
+    const bool a2 = ez | (s2 == 0 ? fin : (E2 - Ee > margin * (fmaxf(fabsf(tn2), fabsf(E2)) + me)));
+    float Te =
+        chain_at<L>(pick(ex, ey, tn0, tn1, tn2), pick(ex, ey, td0, td1, td2), pick(ex, ey, k0, k1, k2) - 1);
     if (!(a0 & a1 & a2)) {
-      const float T0 = chain_at(tn0, td0, k0 - 1), T1 = chain_at(tn1, td1, k1 - 1), T2 = chain_at(tn2, td2, k2 - 1);
+      const float T0 = chain_at<L>(tn0, td0, k0 - 1), T1 = chain_at<L>(tn1, td1, k1 - 1),
+                  T2 = chain_at<L>(tn2, td2, k2 - 1);
       ex = (T0 <= T1) & (T0 <= T2);
       ey = !ex & (T1 <= T2);
       ez = !ex & !ey;
@@ -377,9 +417,18 @@ struct Walk {
     // The other two axes, a < b (e = 0: 1, 2; e = 1: 0, 2; e = 2: 0, 1): steps
     // at chain values before (Te, e); an axis below e also takes the steps that
     // tie with Te (a unless e = 0, b when e = 2).
+    const float ta = ex ? tn1 : tn0, da = ex ? td1 : td0, ra = ex ? rtd1 : rtd0;
+    const float tb = ez ? tn1 : tn2, db = ez ? td1 : td2, rb = ez ? rtd1 : rtd2;
+    const int ka = ex ? k1 : k0, kb = ez ? k1 : k2;
+    const bool mova = (ex ? s1 : s0) != 0, movb = (ez ? s1 : s2) != 0;
     int na, nb;
-    const float ca = chain_before(ex ? tn1 : tn0, ex ? td1 : td0, ex ? k1 : k0, Te, !ex, &na);
-    const float cb = chain_before(ez ? tn1 : tn2, ez ? td1 : td2, ez ? k1 : k2, Te, ez, &nb);
+    float ca, cb;
+    const bool fa = count_near<L>(ta, da, ra, ka, Te, !ex, margin, mova, &na, &ca);
+    const bool fb = count_near<L>(tb, db, rb, kb, Te, ez, margin, movb, &nb, &cb);
+    if (!(fa & fb)) {  // tdelta tiny against the ray's scale: value by value
+      ca = chain_before<L>(ta, da, ka, Te, !ex, &na);
+      cb = chain_before<L>(tb, db, kb, Te, ez, &nb);
+    }
     const float Tn = Te + pick(ex, ey, td0, td1, td2);
     i0 = ex ? ne : i0 + na * s0;
     i1 = ey ? ne : i1 + (ex ? na : nb) * s1;
@@ -392,20 +441,54 @@ struct Walk {
     return ok;
   }
 
-  // c after m (0..7) additions of d
+  __host__ __device__ __forceinline__ bool leave_brick() { return leave_box<8>(); }
+
+  // The number of values of the chain c, c + d, ... (its first k) that come
+  // before T (below it, or equal when tie), and the value after that many
+  // additions.  Chain value j is within (L + 3) * 2^-24 * s of c + j d, and
+  // x = (T - c) / d computed here within 3 * 2^-24 * s / d of the exact
+  // quotient, s = |c| + |T| + L d; so with d above twice `margin` * s, value
+  // n - 2 is below T and value n + 1 above it for n = ceil(x) (clamped to
+  // 0..k-1), and the count is n - 1, n or n + 1, settled by comparing values
+  // n - 1 and n with T.  One chain of additions yields both.  False (nothing
+  // set) when d is not that large.  A still axis (d = c = +inf) takes no step.
+  template <int L>
+  static __host__ __device__ __forceinline__ bool count_near(float c, float d, float rd, int k, float T, bool tie,
+                                                             float margin, bool moving, int *count, float *value) {
+    const float x = (T - c) * rd;
+    const int n = moving ? (int)fminf(fmaxf(ceilf(x), 0.0f), (float)(k - 1)) : 0;
+    float cur = c, prev = c;
+#pragma unroll
+    for (int j = 1; j < L; j++) {
+      const bool upd = j <= n;
+      const float nc = cur + d;
+      prev = upd ? cur : prev;
+      cur = upd ? nc : cur;
+    }
+    const float next = cur + d;
+    const bool pp = (n == 0) | (prev < T) | (tie & (prev == T));  // value n - 1 comes before T
+    const bool pc = (cur < T) | (tie & (cur == T));               // value n does
+    *count = moving ? n - 1 + (int)pp + (int)pc : 0;
+    *value = moving ? (!pp ? prev : (pc ? next : cur)) : c;
+    return !moving | (d > 2.0f * margin * (fabsf(c) + fabsf(T) + (float)L * d));
+  }
+
+  // c after m (0..L-1) additions of d
+  template <int L>
   static __host__ __device__ __forceinline__ float chain_at(float c, float d, int m) {
 #pragma unroll
-    for (int j = 1; j < 8; j++) c = j <= m ? c + d : c;
+    for (int j = 1; j < L; j++) c = j <= m ? c + d : c;
     return c;
   }
 
-  // The chain c, c + d, ... over its first k values: *n = how many come before
-  // T (or equal it when tie), and the value after that many additions.
+  // The chain c, c + d, ... over its first k (<= L) values: *n = how many come
+  // before T (or equal it when tie), and the value after that many additions.
+  template <int L>
   static __host__ __device__ __forceinline__ float chain_before(float c, float d, int k, float T, bool tie, int *n) {
     float r = c;
     int cnt = 0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < L; j++) {
       const bool before = (j < k) & ((c < T) | (tie & (c == T)));  // & and |: no branches
       const float c1 = c + d;
       r = before ? c1 : r;
@@ -434,8 +517,9 @@ struct Walk {
 // the outer loop the same way.
 __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, RenderCam cam,
                                                   int W, int H, float *rgb, u64 *hit, u8 *face,
-                                                  const u64 *__restrict__ coarse, const u64 *__restrict__ fine, int bnx,
-                                                  int bny, int bwords) {
+                                                  const u64 *__restrict__ coarse, const u64 *__restrict__ fine,
+                                                  const u64 *__restrict__ region, int bnx, int bny, int bwords, int rny,
+                                                  int rwords) {
   const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= W || py >= H) return;
   const size_t pix = (size_t)py * W + px;
@@ -501,7 +585,21 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
     bool found = false;
     size_t hli = 0;
     u32 hm = 0;
+    long long crw = -1;
+    u64 crword = 0;
     for (int nb = 0; nb < maxb; nb++) {
+      if (jump) {  // an all-EMPTY 32^3 region is left in one jump
+        const int rx = wk.i0 >> 5, ry = wk.i1 >> 5, rz = wk.i2 >> 5;
+        const long long rw = ((long long)rz * rny + ry) * rwords + (rx >> 6);
+        if (rw != crw) {
+          crw = rw;
+          crword = region[rw];
+        }
+        if (!((crword >> (rx & 63)) & 1ull)) {
+          if (!wk.leave_box<32>()) break;  // the walk ended: a miss
+          continue;
+        }
+      }
       const int ux = wk.i0 >> 3, uy = wk.i1 >> 3, uz = wk.i2 >> 3;
       const long long w = ((long long)uz * bny + uy) * bwords + (ux >> 6);
       if (w != cw) {  // rays along x reuse one coarse word for 64 bricks
@@ -585,8 +683,12 @@ static size_t coarse_words(int nx, int ny, int nz) {
   return bwords * bricks(ny) * bricks(nz);
 }
 
+static size_t regions(int n) { return (bricks(n) + 3) / 4; }  // 32-cell regions along an axis of n cells
+
+static size_t region_words(int nx, int ny, int nz) { return (regions(nx) + 63) / 64 * regions(ny) * regions(nz); }
+
 size_t occ_words(int nx, int ny, int nz) {
-  return coarse_words(nx, ny, nz) + bricks(nx) * bricks(ny) * bricks(nz);
+  return coarse_words(nx, ny, nz) + bricks(nx) * bricks(ny) * bricks(nz) + region_words(nx, ny, nz);
 }
 
 size_t dirty_words(int nx, int ny, int nz) { return coarse_words(nx, ny, nz); }
@@ -605,11 +707,16 @@ hipError_t launch_dda(const u8 *g, int nx, int ny, int nz, const RenderCam &cam,
   const int bnx = (int)bricks(nx), bny = (int)bricks(ny), bnz = (int)bricks(nz), bwords = (bnx + 63) / 64;
   const long long waves = (long long)bwords * bny * bnz;
   u64 *coarse = d_occ, *fine = d_occ + coarse_words(nx, ny, nz);
+  u64 *region = fine + (size_t)bnx * bny * bnz;
+  const int rnx = (int)regions(nx), rny = (int)regions(ny), rnz = (int)regions(nz), rwords = (rnx + 63) / 64;
   hipLaunchKernelGGL(occ_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, g, nx, ny, nz, bnx, bny, bnz,
                      bwords, coarse, fine, d_dirty, update ? 1 : 0);
+  const long long rwaves = (long long)rwords * rny * rnz;
+  hipLaunchKernelGGL(region_kernel, dim3((unsigned)((rwaves + 3) / 4)), dim3(256), 0, s, coarse, bny, bnz, bwords,
+                     rnx, rny, rnz, rwords, region);
   const dim3 grid((W + 15) / 16, (H + 15) / 16);
   hipLaunchKernelGGL(dda_kernel, grid, dim3(256), 0, s, g, nx, ny, nz, cam, W, H, d_rgb, d_hit, d_face, coarse, fine,
-                     bnx, bny, bwords);
+                     region, bnx, bny, bwords, rny, rwords);
   return hipGetLastError();
 }
 

@@ -368,15 +368,18 @@ int fs3d_world_get_render_track(const fs3d_world *w, int32_t *mode) {
   return clear_error();
 }
 
-int fs3d_world_render_maps(fs3d_world *w, uint64_t *coarse, size_t n_coarse, uint64_t *fine, size_t n_fine) {
-  if (!w || !coarse || !fine) return set_error(FS3D_ERR_INVALID, "NULL argument");
+int fs3d_world_render_maps(fs3d_world *w, uint64_t *coarse, size_t n_coarse, uint64_t *fine, size_t n_fine,
+                           uint64_t *region, size_t n_region) {
+  if (!w || !coarse || !fine || !region) return set_error(FS3D_ERR_INVALID, "NULL argument");
   if (!w->d_occ) return set_error(FS3D_ERR_STATE, "the world has not been rendered");
-  const size_t nc = dirty_words(w->nx, w->ny, w->nz), nf = occ_words(w->nx, w->ny, w->nz) - nc;
-  if (n_coarse < nc || n_fine < nf)
-    return set_error(FS3D_ERR_INVALID, "maps need %zu coarse and %zu fine words", nc, nf);
+  const size_t nb = ((size_t)w->nx + 7) / 8 * (((size_t)w->ny + 7) / 8) * (((size_t)w->nz + 7) / 8);
+  const size_t nc = dirty_words(w->nx, w->ny, w->nz), nr = occ_words(w->nx, w->ny, w->nz) - nc - nb;
+  if (n_coarse < nc || n_fine < nb || n_region < nr)
+    return set_error(FS3D_ERR_INVALID, "maps need %zu coarse, %zu fine and %zu region words", nc, nb, nr);
   FS3D_HIP(hipSetDevice(w->dev));
   FS3D_HIP(hipMemcpyAsync(coarse, w->d_occ, nc * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
-  FS3D_HIP(hipMemcpyAsync(fine, w->d_occ + nc, nf * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipMemcpyAsync(fine, w->d_occ + nc, nb * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
+  FS3D_HIP(hipMemcpyAsync(region, w->d_occ + nc + nb, nr * sizeof(u64), hipMemcpyDeviceToHost, w->stream));
   FS3D_HIP(hipStreamSynchronize(w->stream));
   return clear_error();
 }

@@ -90,14 +90,17 @@ def last_render_ms(world):
 
 
 def render_maps(world):
-    """(coarse, fine) occupancy maps of the world's last render, as uint64
-    arrays in the layouts fs3d_world_render_maps documents."""
+    """(coarse, fine, region) occupancy maps of the world's last render, as
+    uint64 arrays in the layouts fs3d_world_render_maps documents."""
     nx, ny, nz = world.dims
     bnx, bny, bnz = (nx + 7) // 8, (ny + 7) // 8, (nz + 7) // 8
+    rnx, rny, rnz = (bnx + 3) // 4, (bny + 3) // 4, (bnz + 3) // 4
     coarse = np.zeros(((bnx + 63) // 64) * bny * bnz, np.uint64)
     fine = np.zeros(bnx * bny * bnz, np.uint64)
-    check(lib().fs3d_world_render_maps(world.handle, ptr(coarse, c_u64), coarse.size, ptr(fine, c_u64), fine.size))
-    return coarse, fine
+    region = np.zeros(((rnx + 63) // 64) * rny * rnz, np.uint64)
+    check(lib().fs3d_world_render_maps(world.handle, ptr(coarse, c_u64), coarse.size, ptr(fine, c_u64), fine.size,
+                                       ptr(region, c_u64), region.size))
+    return coarse, fine, region
 
 
 def camera_globaldata(cam_pos, cam_rot):

@@ -262,14 +262,19 @@ int fs3d_world_set_render_track(fs3d_world *w, int32_t mode);
 int fs3d_world_get_render_track(const fs3d_world *w, int32_t *mode);
 /* Diagnostics (no counterpart in the reference): the occupancy maps of the last
  * render, which describe the grid as it was then.  With bnx = ceil(nx / 8) (and
- * likewise bny, bnz) and bw = ceil(bnx / 64):
- *   coarse[(bz * bny + by) * bw + bx / 64] bit bx % 64: brick (bx, by, bz) holds
- *     a non-EMPTY cell;
+ * likewise bny, bnz), bw = ceil(bnx / 64), rnx = ceil(bnx / 4) (and likewise
+ * rny, rnz) and rw = ceil(rnx / 64):
+ *   coarse[(bz * bny + by) * bw + bx / 64] bit bx % 64: the 8^3 brick (bx, by, bz)
+ *     holds a non-EMPTY cell;
  *   fine[(bz * bny + by) * bnx + bx] bit ((x & 7) >> 1) | (((y & 7) >> 1) << 2) |
- *     (((z & 7) >> 1) << 4): the 2x2x2 sub-brick holding (x, y, z) does.
- * n_coarse and n_fine are the capacities in words (at least bw * bny * bnz and
- * bnx * bny * bnz).  FS3D_ERR_STATE before the first render.  Synchronises. */
-int fs3d_world_render_maps(fs3d_world *w, uint64_t *coarse, size_t n_coarse, uint64_t *fine, size_t n_fine);
+ *     (((z & 7) >> 1) << 4): the 2x2x2 sub-brick holding (x, y, z) does;
+ *   region[(rz * rny + ry) * rw + rx / 64] bit rx % 64: the 32^3 region (rx, ry,
+ *     rz) does.
+ * n_coarse, n_fine and n_region are the capacities in words (at least
+ * bw * bny * bnz, bnx * bny * bnz and rw * rny * rnz).  FS3D_ERR_STATE before the
+ * first render.  Synchronises. */
+int fs3d_world_render_maps(fs3d_world *w, uint64_t *coarse, size_t n_coarse, uint64_t *fine, size_t n_fine,
+                           uint64_t *region, size_t n_region);
 /* Seeded-orbit camera: eye at angle/elevation (radians) around the grid centre,
  * dist_scale * max(dims) away, looking at the centre with +z up on screen. */
 int fs3d_orbit_camera(const int32_t dims[3], double angle, double dist_scale, double elev, float aspect,

@@ -1,7 +1,7 @@
-// Host-side check of the voxel ray caster's brick jump: for random rays over
-// random grid sizes, Walk::leave_brick() must leave every brick in exactly the
-// state (cell, tnext, t, face, end of walk) that Walk::advance() reaches one
-// step at a time.  The rays include axis-aligned and diagonal directions and
+// Host-side check of the voxel ray caster's box jumps: for random rays over
+// random grid sizes, Walk::leave_box<L>() must leave every 8^3 brick (L = 8) and
+// every 32^3 region (L = 32) in exactly the state (cell, tnext, t, face, end of
+// walk) that Walk::advance() reaches one step at a time.  The rays include axis-aligned and diagonal directions and
 // integer origins, which make tnext values tie across axes.
 //
 // Usage: walk_jump_check [trials] [seed]; prints one summary line, exit 1 on a
@@ -71,15 +71,41 @@ bool start_walk(const float o[3], const float d[3], int nx, int ny, int nz, Walk
   return true;
 }
 
+// Walks the whole ray box by box (L = 8: bricks, L = 32: regions): each box
+// left once by Walk::leave_box<L>() and once by single steps, then compared.
+template <int L>
+void walk_boxes(Walk w, const int n[3], const float o[3], const float d[3], long long *boxes, long long *ties,
+                long long *bad) {
+  constexpr int sh = L == 8 ? 3 : 5;
+  for (;;) {
+    Walk a = w, b = w;
+    bool crossed = false, go;
+    do {
+      go = a.advance(crossed);
+    } while (go && (a.i0 >> sh) == (w.i0 >> sh) && (a.i1 >> sh) == (w.i1 >> sh) && (a.i2 >> sh) == (w.i2 >> sh));
+    const bool gb = b.leave_box<L>();
+    (*boxes)++;
+    *ties += go && (a.tn0 == a.tn1 || a.tn1 == a.tn2 || a.tn0 == a.tn2);
+    if (go != gb || (go && !same(a, b))) {
+      if ((*bad)++ < 5)
+        std::printf("mismatch (L=%d): dims %d %d %d o %.9g %.9g %.9g d %.9g %.9g %.9g\n", L, n[0], n[1], n[2], o[0],
+                    o[1], o[2], d[0], d[1], d[2]);
+      return;
+    }
+    if (!go) return;
+    w = a;
+  }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   const long trials = argc > 1 ? std::atol(argv[1]) : 100000;
   std::mt19937_64 rng(argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 1);
   std::uniform_real_distribution<float> U(0.0f, 1.0f);
-  long long rays = 0, bricks = 0, ties = 0, bad = 0;
+  long long rays = 0, bricks = 0, regions = 0, ties = 0, bad = 0;
   for (long trial = 0; trial < trials; trial++) {
-    const int n[3] = {1 + (int)(rng() % 70), 1 + (int)(rng() % 70), 1 + (int)(rng() % 70)};
+    const int n[3] = {1 + (int)(rng() % 150), 1 + (int)(rng() % 150), 1 + (int)(rng() % 150)};
     const int mode = (int)(rng() % 8);
     float o[3], dd[3], d[3];
     for (int a = 0; a < 3; a++)  // integer and half-integer origins, or anywhere around the grid
@@ -102,25 +128,9 @@ int main(int argc, char **argv) {
     Walk w;
     if (!start_walk(o, d, n[0], n[1], n[2], &w) || !w.no_nan()) continue;
     rays++;
-    for (;;) {
-      Walk a = w, b = w;
-      bool crossed = false, go;
-      do {
-        go = a.advance(crossed);
-      } while (go && !crossed);
-      const bool gb = b.leave_brick();
-      bricks++;
-      ties += go && (a.tn0 == a.tn1 || a.tn1 == a.tn2 || a.tn0 == a.tn2);
-      if (go != gb || (go && !same(a, b))) {
-        if (bad++ < 5)
-          std::printf("mismatch: dims %d %d %d o %.9g %.9g %.9g d %.9g %.9g %.9g\n", n[0], n[1], n[2], o[0], o[1],
-                      o[2], d[0], d[1], d[2]);
-        break;
-      }
-      if (!go) break;
-      w = a;
-    }
+    walk_boxes<8>(w, n, o, d, &bricks, &ties, &bad);
+    walk_boxes<32>(w, n, o, d, &regions, &ties, &bad);
   }
-  std::printf("rays %lld bricks %lld tied %lld mismatches %lld\n", rays, bricks, ties, bad);
+  std::printf("rays %lld bricks %lld regions %lld tied %lld mismatches %lld\n", rays, bricks, regions, ties, bad);
   return bad != 0;
 }

@@ -24,7 +24,8 @@ def pal():
 
 def expected_maps(g):
     """The occupancy maps of grid g[z][y][x] (numpy, from their definition):
-    coarse bits per 8^3 brick and fine words with a bit per 2x2x2 sub-brick."""
+    coarse bits per 8^3 brick, fine words with a bit per 2x2x2 sub-brick, and
+    region bits per 32^3 region."""
     nz, ny, nx = g.shape
     bnx, bny, bnz = -(-nx // 8), -(-ny // 8), -(-nz // 8)
     pad = np.zeros((bnz * 8, bny * 8, bnx * 8), bool)
@@ -38,7 +39,16 @@ def expected_maps(g):
     coarse = np.zeros(bnz * bny * bw, np.uint64)
     for bx in range(bnx):
         coarse[np.arange(bnz * bny) * bw + bx // 64] |= occ[:, bx].astype(np.uint64) << np.uint64(bx % 64)
-    return coarse, fine
+    # regions of 4 x 4 x 4 bricks
+    rnx, rny, rnz = -(-bnx // 4), -(-bny // 4), -(-bnz // 4)
+    rpad = np.zeros((rnz * 4, rny * 4, rnx * 4), bool)
+    rpad[:bnz, :bny, :bnx] = (fine != 0).reshape(bnz, bny, bnx)
+    rocc = rpad.reshape(rnz, 4, rny, 4, rnx, 4).any(axis=(1, 3, 5)).reshape(rnz * rny, rnx)
+    rw = -(-rnx // 64)
+    region = np.zeros(rnz * rny * rw, np.uint64)
+    for rx in range(rnx):
+        region[np.arange(rnz * rny) * rw + rx // 64] |= rocc[:, rx].astype(np.uint64) << np.uint64(rx % 64)
+    return coarse, fine, region
 
 
 def _frame(R, O, w, cam, pal, W, H, label):
@@ -46,10 +56,11 @@ def _frame(R, O, w, cam, pal, W, H, label):
     g = w.download()
     # the maps after a render describe the current grid exactly, however they
     # were brought up to date
-    coarse, fine = R.render_maps(w)
-    want_c, want_f = expected_maps(g)
+    coarse, fine, region = R.render_maps(w)
+    want_c, want_f, want_r = expected_maps(g)
     assert (fine == want_f).all(), (label, "fine", int((fine != want_f).sum()))
     assert (coarse == want_c).all(), (label, "coarse")
+    assert (region == want_r).all(), (label, "region")
     ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
     wrgb, whit, wface = O.render_dda(g, ocam, pal, W, H)
     assert (hit == whit).all(), (label, int((hit != whit).sum()))
