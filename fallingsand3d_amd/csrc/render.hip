@@ -457,14 +457,9 @@ struct Walk {
                                                              float margin, bool moving, int *count, float *value) {
     const float x = (T - c) * rd;
     const int n = moving ? (int)fminf(fmaxf(ceilf(x), 0.0f), (float)(k - 1)) : 0;
-    float cur = c, prev = c;
-#pragma unroll
-    for (int j = 1; j < L; j++) {
-      const bool upd = j <= n;
-      const float nc = cur + d;
-      prev = upd ? cur : prev;
-      cur = upd ? nc : cur;
-    }
+    // value n - 1 by the chain; value n is one more of its additions
+    const float prev = chain_at<L>(c, d, n - 1);
+    const float cur = n == 0 ? c : prev + d;
     const float next = cur + d;
     const bool pp = (n == 0) | (prev < T) | (tie & (prev == T));  // value n - 1 comes before T
     const bool pc = (cur < T) | (tie & (cur == T));               // value n does
@@ -473,7 +468,7 @@ struct Walk {
     return !moving | (d > 2.0f * margin * (fabsf(c) + fabsf(T) + (float)L * d));
   }
 
-  // c after m (0..L-1) additions of d
+  // c after m (at most L - 1; none when m <= 0) additions of d
   template <int L>
   static __host__ __device__ __forceinline__ float chain_at(float c, float d, int m) {
 #pragma unroll
