@@ -499,17 +499,20 @@ struct Walk {
 // traversal: at each cell, test it, then take the axis with the smallest
 // tnext (ties x, then y, then z), t = tnext[a], idx[a] += step[a] (leaving the
 // grid or a zero step ends the walk), tnext[a] += tdelta[a].  It is organised
-// per brick: on entering a brick the coarse bit and, when set, the fine word
-// are read once; a cell is loaded only when its sub-brick bit is set, every
-// other cell is EMPTY by construction of the maps.  The stepping arithmetic is
-// the same sequence of float operations whatever the maps say, so t, the hit
-// cell and the face are those of the plain walk.
+// per brick: on entering a brick the region bit, then the coarse bit and, when
+// set, the fine word are read once.  An all-EMPTY 32^3 region or 8^3 brick is
+// left in one jump (Walk::leave_box); in an occupied brick a cell is loaded
+// only when its sub-brick bit is set, every other cell is EMPTY by
+// construction of the maps.  The jumps reproduce the single steps' float
+// operations exactly, so t, the hit cell and the face are those of the plain
+// walk.
 //
 // Bounds: every step moves one axis by one cell in that axis's fixed direction,
 // so a walk ends after at most nx + ny + nz steps, and a brick is left after at
 // most 7 in-brick steps per axis.  The plain walk's cap of nx + ny + nz + 3
-// tested cells is therefore never reached, and the per-brick count below caps
-// the outer loop the same way.
+// tested cells is therefore never reached, and the outer loop, each iteration
+// of which leaves at least one brick, is capped by the number of bricks a
+// walk can cross.
 __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int nx, int ny, int nz, RenderCam cam,
                                                   int W, int H, float *rgb, u64 *hit, u8 *face,
                                                   const u64 *__restrict__ coarse, const u64 *__restrict__ fine,
