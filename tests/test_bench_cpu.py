@@ -1,0 +1,76 @@
+"""CPU checks of bench.py's host-side figures: the voxel-steps count of a frame
+must equal the cells the cell-by-cell walk visits (within the one-cell slack the
+estimate documents at cell faces)."""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def _walk_cells(g, cam, W, H):
+    """Cells visited per frame by the single-step walk, in fp32 as the kernel
+    sets it up (slab test, start cell, tnext chains, ties x then y then z)."""
+    nz, ny, nx = g.shape
+    dims = (nx, ny, nz)
+    f32 = np.float32
+    b = np.array(cam.basis[:], f32).reshape(3, 3)
+    o = np.array(cam.origin[:], f32)
+    total = 0
+    for py in range(H):
+        for px in range(W):
+            ux = (f32(W - px) - f32(0.5)) / f32(W) * f32(2) - f32(1)
+            uy = ((f32(py) + f32(0.5)) / f32(H) * f32(2) - f32(1)) * (f32(1) / f32(cam.aspect))
+            l = np.sqrt(f32(ux * ux + uy * uy) + f32(1))
+            dc = np.array([ux / l, uy / l, f32(1) / l], f32)
+            d = ((dc[0] * b[0] + dc[1] * b[1]) + dc[2] * b[2]).astype(f32)
+            d = (d / np.sqrt(f32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])).astype(f32)
+            tmin, tmax, miss = f32(0), f32(np.inf), False
+            for a in range(3):
+                if d[a] == 0:
+                    miss |= bool(o[a] < 0 or o[a] > dims[a])
+                    continue
+                t1 = (f32(0) - o[a]) / d[a]
+                t2 = (f32(dims[a]) - o[a]) / d[a]
+                tmin, tmax = max(tmin, min(t1, t2)), min(tmax, max(t1, t2))
+            if miss or tmin > tmax:
+                continue
+            idx = [min(max(int(np.floor(o[a] + tmin * d[a])), 0), dims[a] - 1) for a in range(3)]
+            st = [1 if d[a] > 0 else (-1 if d[a] < 0 else 0) for a in range(3)]
+            td = [abs(f32(1) / d[a]) if st[a] else f32(np.inf) for a in range(3)]
+            tn = [((f32(idx[a] + 1) - o[a]) / d[a]) if st[a] > 0 else
+                  (((f32(idx[a]) - o[a]) / d[a]) if st[a] < 0 else f32(np.inf)) for a in range(3)]
+            while True:
+                total += 1
+                if g[idx[2], idx[1], idx[0]]:
+                    break
+                a = 0 if (tn[0] <= tn[1] and tn[0] <= tn[2]) else (1 if tn[1] <= tn[2] else 2)
+                if st[a] == 0:
+                    break
+                idx[a] += st[a]
+                if not 0 <= idx[a] < dims[a]:
+                    break
+                tn[a] = f32(tn[a] + td[a])
+    return total
+
+
+def test_voxel_steps_counts_the_walked_cells(O):
+    import bench
+    from fallingsand3d_amd import render as R
+    pal = np.array(json.load(open(os.path.join(ROOT, "tests", "golden", "palette.json")))["rgba"], np.float32)
+    rng = np.random.default_rng(4)
+    for dims, density in [((37, 29, 23), 0.01), ((20, 33, 17), 0.05)]:
+        nx, ny, nz = dims
+        g = (rng.random((nz, ny, nx)) < density).astype(np.uint8) * 2
+        for ang in (0.3, 2.0, 4.4):
+            W, H = 32, 24
+            cam = R.orbit_camera(dims, ang, 1.3, 0.3, W / H)
+            ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
+            _, hit, _ = O.render_dda(g, ocam, pal, W, H)
+            est = bench.voxel_steps(cam, dims, hit, W, H)
+            exact = _walk_cells(g, cam, W, H)
+            assert abs(est - exact) <= max(2, exact // 200), (dims, ang, est, exact)
