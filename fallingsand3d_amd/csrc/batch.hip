@@ -43,6 +43,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *or
     if (idx >= n) break;
     const int wi = order[idx];
     const BWorld w = worlds[wi];
+    u8 *const g = grid_of(w);
     u64 acc = 0;
     int quiet = 0;  // consecutive ticks in which no cell of the world changed
     for (int k = 0; k < nticks; k++) {
@@ -61,10 +62,10 @@ __global__ __launch_bounds__(NT) void batch_kernel(BWorld *worlds, const int *or
       // each wave takes NR row-quad groups per step (wv, wv + NW, ...)
       if (want_hash && k == nticks - 1) {
         for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
-          acc += tick_waves<S, true, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
+          acc += tick_waves<S, true, NR>(g, g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
       } else {
         for (long long wv = wid; wv < nwav; wv += (long long)NW * NR)
-          tick_waves<S, false, NR>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
+          tick_waves<S, false, NR>(g, g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wv, NW, lane, rule, &ch);
       }
       quiet = __syncthreads_or(ch) ? 0 : quiet + 1;
     }
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void batch_hash_kernel(const BWorld *worlds, i
     const size_t cnt = (size_t)w.nx * w.ny * w.nz;
     u64 acc = 0;
     for (size_t i = threadIdx.x; i < cnt; i += 256) {
-      const u32 v = w.g[i];
+      const u32 v = grid_of(w)[i];
       if (v) acc += dmix64(((u64)i << 8) | v);
     }
     acc = wave_sum_u64(acc);
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void batch_fill_rain_kernel(const BWorld *worl
         v = FS3D_STATIC;
       else if (z >= (size_t)zr && dmix64(seed + (i + 1) * kGamma) < th)
         v = FS3D_SAND;
-      w.g[i] = v;
+      grid_of(w)[i] = v;
     }
   }
 }

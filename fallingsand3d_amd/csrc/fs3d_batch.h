@@ -16,6 +16,14 @@ struct BWorld {
   u64 tick;      // tick counter before the next run (round)
 };
 
+// The grid of a world-table entry as a global-memory pointer.  A pointer read
+// from a table is generic to the compiler, which then emits flat loads and
+// stores (they also wait on the LDS counter); passed through an integer into
+// an address_space(1) pointer, it lets the compiler emit global ones.
+__device__ __forceinline__ u8 *grid_of(const BWorld &w) {
+  return (u8 *)(__attribute__((address_space(1))) u8 *)(uintptr_t)w.g;
+}
+
 bool batch_config_ok(int seg, int threads, int groups);
 int batch_occupancy(int seg, int threads, int groups);
 // flags of the persistent batch launch: hash every world's final tick; stop

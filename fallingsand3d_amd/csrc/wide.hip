@@ -45,11 +45,11 @@ __global__ __launch_bounds__(256) void wide_kernel(const BWorld *__restrict__ en
   const int lane = threadIdx.x & 63;
   bool ch = false;
   if (final_hash) {
-    u64 acc = tick_wave<S, true, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
+    u64 acc = tick_wave<S, true, true>(grid_of(w), grid_of(w), w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
     acc = wave_sum_u64(acc);
     if (lane == 0 && acc) atomicAdd((unsigned long long *)(hashes + w.slot), (unsigned long long)acc);
   } else {
-    tick_wave<S, false, true>(w.g, w.g, w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
+    tick_wave<S, false, true>(grid_of(w), grid_of(w), w.nx, w.ny, w.nz, w.lr, w.rqpw, t, wave, lane, rule, &ch);
   }
   if (flags && __ballot(ch) && lane == 0) flags[t & 3u] = 1;
 }
