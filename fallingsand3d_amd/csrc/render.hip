@@ -371,13 +371,16 @@ struct Walk {
   // when it is least, advance() would pick it and end the walk.  The additions
   // are the same ones advance() makes, so t, tnext and the indices come out
   // bit for bit as after the single steps.
+  //
+  // The side is lm + 1 (8 or 32, up to L, at run time: lanes leaving bricks and
+  // regions share one pass); the chains run up to L - 1 additions.
   template <int L>
-  __host__ __device__ __forceinline__ bool leave_box() {
+  __host__ __device__ __forceinline__ bool leave_box(int lm = L - 1) {
     // steps to leave the box's part of the grid along each axis (on a still
-    // axis any value in 1..L: its chain is +inf throughout)
-    const int k0 = min((i0 ^ (up0 & (L - 1))) & (L - 1), hi0 - i0) + 1;
-    const int k1 = min((i1 ^ (up1 & (L - 1))) & (L - 1), hi1 - i1) + 1;
-    const int k2 = min((i2 ^ (up2 & (L - 1))) & (L - 1), hi2 - i2) + 1;
+    // axis any value in 1..lm+1: its chain is +inf throughout)
+    const int k0 = min((i0 ^ (up0 & lm)) & lm, hi0 - i0) + 1;
+    const int k1 = min((i1 ^ (up1 & lm)) & lm, hi1 - i1) + 1;
+    const int k2 = min((i2 ^ (up2 & lm)) & lm, hi2 - i2) + 1;
     // The leaving axis e and its leaving time Te.  First from estimates: on a
     // moving axis E = tnext + (k - 1) * tdelta is within (L + 2) * 2^-24 *
     // max(|tnext|, |E|) of the chain value T (k - 1 <= L - 1 rounded additions
@@ -440,8 +443,6 @@ struct Walk {
     f = pick(ex, ey, fc0, fc1, fc2);
     return ok;
   }
-
-  __host__ __device__ __forceinline__ bool leave_brick() { return leave_box<8>(); }
 
   // The number of values of the chain c, c + d, ... (its first k) that come
   // before T (below it, or equal when tie), and the value after that many
@@ -586,28 +587,29 @@ __global__ __launch_bounds__(256) void dda_kernel(const u8 *__restrict__ g, int 
     long long crw = -1;
     u64 crword = 0;
     for (int nb = 0; nb < maxb; nb++) {
-      if (jump) {  // an all-EMPTY 32^3 region is left in one jump
+      bool empty_region = false;
+      if (jump) {
         const int rx = wk.i0 >> 5, ry = wk.i1 >> 5, rz = wk.i2 >> 5;
         const long long rw = ((long long)rz * rny + ry) * rwords + (rx >> 6);
         if (rw != crw) {
           crw = rw;
           crword = region[rw];
         }
-        if (!((crword >> (rx & 63)) & 1ull)) {
-          if (!wk.leave_box<32>()) break;  // the walk ended: a miss
-          continue;
+        empty_region = !((crword >> (rx & 63)) & 1ull);
+      }
+      u64 fw = 0;
+      if (!empty_region) {
+        const int ux = wk.i0 >> 3, uy = wk.i1 >> 3, uz = wk.i2 >> 3;
+        const long long w = ((long long)uz * bny + uy) * bwords + (ux >> 6);
+        if (w != cw) {  // rays along x reuse one coarse word for 64 bricks
+          cw = w;
+          cword = coarse[w];
         }
+        fw = ((cword >> (ux & 63)) & 1ull) ? fine[((long long)uz * bny + uy) * bnx + ux] : 0ull;
       }
-      const int ux = wk.i0 >> 3, uy = wk.i1 >> 3, uz = wk.i2 >> 3;
-      const long long w = ((long long)uz * bny + uy) * bwords + (ux >> 6);
-      if (w != cw) {  // rays along x reuse one coarse word for 64 bricks
-        cw = w;
-        cword = coarse[w];
-      }
-      const u64 fw = ((cword >> (ux & 63)) & 1ull) ? fine[((long long)uz * bny + uy) * bnx + ux] : 0ull;
       bool crossed = false, go = true;
-      if (fw == 0 && jump) {
-        go = wk.leave_brick();
+      if (fw == 0 && jump) {  // an all-EMPTY 32^3 region or 8^3 brick, left in one jump
+        go = wk.leave_box<32>(empty_region ? 31 : 7);
       } else if (fw == 0) {  // rays whose chains could reach NaN: the single steps
         do {
           go = wk.advance(crossed);

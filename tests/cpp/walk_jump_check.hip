@@ -1,7 +1,7 @@
 // Host-side check of the voxel ray caster's box jumps: for random rays over
-// random grid sizes, Walk::leave_box<L>() must leave every 8^3 brick (L = 8) and
-// every 32^3 region (L = 32) in exactly the state (cell, tnext, t, face, end of
-// walk) that Walk::advance() reaches one step at a time.  The rays include axis-aligned and diagonal directions and
+// random grid sizes, Walk::leave_box must leave every 8^3 brick and every 32^3
+// region in exactly the state (cell, tnext, t, face, end of walk) that
+// Walk::advance() reaches one step at a time.  The rays include axis-aligned and diagonal directions and
 // integer origins, which make tnext values tie across axes.
 //
 // Usage: walk_jump_check [trials] [seed]; prints one summary line, exit 1 on a
@@ -71,25 +71,26 @@ bool start_walk(const float o[3], const float d[3], int nx, int ny, int nz, Walk
   return true;
 }
 
-// Walks the whole ray box by box (L = 8: bricks, L = 32: regions): each box
-// left once by Walk::leave_box<L>() and once by single steps, then compared.
+// Walks the whole ray box by box (side lm + 1: 8 for bricks, 32 for regions):
+// each box left once by Walk::leave_box<L>(lm) and once by single steps, then
+// compared.  L is the longest chain the jump is compiled for.
 template <int L>
-void walk_boxes(Walk w, const int n[3], const float o[3], const float d[3], long long *boxes, long long *ties,
-                long long *bad) {
-  constexpr int sh = L == 8 ? 3 : 5;
+void walk_boxes(Walk w, int lm, const int n[3], const float o[3], const float d[3], long long *boxes,
+                long long *ties, long long *bad) {
+  const int sh = lm == 7 ? 3 : 5;
   for (;;) {
     Walk a = w, b = w;
     bool crossed = false, go;
     do {
       go = a.advance(crossed);
     } while (go && (a.i0 >> sh) == (w.i0 >> sh) && (a.i1 >> sh) == (w.i1 >> sh) && (a.i2 >> sh) == (w.i2 >> sh));
-    const bool gb = b.leave_box<L>();
+    const bool gb = b.leave_box<L>(lm);
     (*boxes)++;
     *ties += go && (a.tn0 == a.tn1 || a.tn1 == a.tn2 || a.tn0 == a.tn2);
     if (go != gb || (go && !same(a, b))) {
       if ((*bad)++ < 5)
-        std::printf("mismatch (L=%d): dims %d %d %d o %.9g %.9g %.9g d %.9g %.9g %.9g\n", L, n[0], n[1], n[2], o[0],
-                    o[1], o[2], d[0], d[1], d[2]);
+        std::printf("mismatch (L=%d, side %d): dims %d %d %d o %.9g %.9g %.9g d %.9g %.9g %.9g\n", L, lm + 1, n[0],
+                    n[1], n[2], o[0], o[1], o[2], d[0], d[1], d[2]);
       return;
     }
     if (!go) return;
@@ -128,8 +129,9 @@ int main(int argc, char **argv) {
     Walk w;
     if (!start_walk(o, d, n[0], n[1], n[2], &w) || !w.no_nan()) continue;
     rays++;
-    walk_boxes<8>(w, n, o, d, &bricks, &ties, &bad);
-    walk_boxes<32>(w, n, o, d, &regions, &ties, &bad);
+    walk_boxes<8>(w, 7, n, o, d, &bricks, &ties, &bad);
+    walk_boxes<32>(w, 7, n, o, d, &bricks, &ties, &bad);  // bricks in the kernel's shared jump pass
+    walk_boxes<32>(w, 31, n, o, d, &regions, &ties, &bad);
   }
   std::printf("rays %lld bricks %lld regions %lld tied %lld mismatches %lld\n", rays, bricks, regions, ties, bad);
   return bad != 0;

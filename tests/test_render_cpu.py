@@ -258,8 +258,9 @@ def test_image_size_limits():
 # ---- the ray caster's brick jump, built for the host ----------------------------------
 
 def test_brick_jump_equals_single_steps(tmp_path):
-    """Walk::leave_brick() (one jump across an all-EMPTY brick) against Walk::advance()
-    taken one step at a time, on 60000 random rays: same cell, tnext, t and face."""
+    """Walk::leave_box (one jump across an all-EMPTY 8^3 brick or 32^3 region)
+    against Walk::advance() taken one step at a time, on 60000 random rays: same
+    cell, tnext, t and face."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -273,5 +274,7 @@ def test_brick_jump_equals_single_steps(tmp_path):
                     os.path.join(root, "tests", "cpp", "walk_jump_check.hip"), "-o", str(exe)], check=True)
     r = subprocess.run([str(exe), "60000", "7"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    rays, bricks, tied = (int(x) for x in r.stdout.split()[1:6:2])
-    assert rays > 30000 and bricks > 100000 and tied > 1000, r.stdout
+    words = r.stdout.split()
+    got = {words[i]: int(words[i + 1]) for i in range(0, len(words) - 1, 2)}
+    assert got["mismatches"] == 0, r.stdout
+    assert got["rays"] > 30000 and got["bricks"] > 100000 and got["regions"] > 30000 and got["tied"] > 1000, r.stdout
