@@ -13,7 +13,7 @@
 //     keys (fs3d_camera_integrate), then renders (here: the voxel ray caster
 //     from fs3d_key_camera, written as a binary PPM).
 //
-// There is no window system in this build, so events come from a script (a
+// The viewer opens no window, so events come from a script (a
 // file, or "-" for stdin), one per line; '#' starts a comment:
 //   down KEY | up KEY    KEY: W A S D CTRL SPACE LEFT RIGHT
 //   frames N             run N frames of the loop with the current key state
