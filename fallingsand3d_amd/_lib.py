@@ -145,6 +145,7 @@ SIGNATURES = {
     "fs3d_file_read": (c_int, [ctypes.c_char_p, vp, c_size]),
     "fs3d_world_save": (c_int, [vp, ctypes.c_char_p]),
     "fs3d_world_load": (c_int, [vp, ctypes.c_char_p]),
+    "fs3d_grid_run_noalloc": (c_int, [vp, vp, P(c_i32), c_u64, c_i32, c_i32, P(c_u64), P(c_i32), vp]),
 }
 
 _LIB = None

@@ -21,7 +21,6 @@ struct DeviceCtx {
   bool ready = false;
   u64 *rule = nullptr;  // 4 x 65536 x 8 bytes
 };
-constexpr int kMaxDevices = 16;
 DeviceCtx g_dev[kMaxDevices];
 std::mutex g_dev_mutex;
 }  // namespace
@@ -60,6 +59,7 @@ int ensure_device(int dev) {
   build_rule_entries(host.data());
   FS3D_HIP(hipMalloc(&c.rule, host.size() * sizeof(u64)));
   FS3D_HIP(hipMemcpy(c.rule, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice));
+  if (int rc = noalloc_setup(dev)) return rc;
   c.ready = true;
   return FS3D_OK;
 }

@@ -26,6 +26,7 @@ constexpr int kHashStride = 8;  // u64 per slot
 // one lane segment (<= 64 bytes) beyond the last row.
 constexpr size_t kGuardBytes = 256;
 constexpr int kMaxNx = 4096;  // 64 lanes x 64-byte segments per row
+constexpr int kMaxDevices = 16;
 
 int set_error(int code, const char *fmt, ...);
 int clear_error();
@@ -65,6 +66,9 @@ int file_read(const char *path, u8 *grid, size_t nbytes, int (*sink)(void *ctx, 
 // ---- device context -------------------------------------------------------------
 int ensure_device(int dev);            // sets the device current, uploads the rule table
 const u64 *device_rule_table(int dev);  // valid after ensure_device
+// Looks up this device's copy of the no-allocation entry point's static scratch
+// (with dev current) and clears it; called once per device by ensure_device.
+int noalloc_setup(int dev);
 
 // ---- tick kernels -----------------------------------------------------------------
 struct TickShape {

@@ -385,6 +385,26 @@ int fs3d_world_save(fs3d_world *w, const char *path);
 /* Loads a file into a world of the same dims: grid, tick counter, checked hashes. */
 int fs3d_world_load(fs3d_world *w, const char *path);
 
+/* ---- ticks without allocation ---------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the ticks of fs3d_world_run (dense, bit-identical results)
+ * on a grid in caller-owned device memory.  a and b are two distinct device
+ * buffers of at least nx*ny*nz + FS3D_GRID_GUARD_BYTES bytes (tick loads may
+ * read up to one lane segment past the last row); the state is read from a, the
+ * buffers alternate tick by tick, and *final_index receives the one holding the
+ * result (0 = a, 1 = b).  hashes[nticks / hash_every] (host memory) receives
+ * the state hash after every hash_every-th tick; at most FS3D_NOALLOC_MAX_HASHES
+ * of them, or FS3D_ERR_INVALID.  The kernels' own device memory (hash partial
+ * sums and hash slots, 96 KiB) is a static array per GPU, set up by fs3d_init
+ * (FS3D_ERR_STATE before it): the call allocates nothing.  stream is a
+ * hipStream_t (NULL: the default stream); the call synchronises it.  Calls on
+ * one GPU share that array: calls that overlap there race. */
+#define FS3D_GRID_GUARD_BYTES 256
+#define FS3D_NOALLOC_MAX_HASHES 4096
+int fs3d_grid_run_noalloc(uint8_t *a, uint8_t *b, const int32_t dims[3], uint64_t tick, int32_t nticks,
+                          int32_t hash_every, uint64_t *hashes, int32_t *final_index, void *stream);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Average device time (ms) of the tick kernels launched by the last
  * fs3d_world_run on this world, measured with HIP events on the world's stream
