@@ -59,7 +59,7 @@ class ServerStats(ctypes.Structure):
                 ("launches", c_i64), ("tick_p50_ms", c_double), ("tick_p99_ms", c_double),
                 ("tick_max_ms", c_double), ("req_p50_ms", c_double), ("req_p99_ms", c_double),
                 ("voxel_updates", c_double), ("busy_ms", c_double), ("peak_bytes", c_i64), ("used_bytes", c_i64),
-                ("peak_active", c_i32), ("reserved", c_i32)]
+                ("peak_active", c_i32), ("reserved", c_i32), ("alloc_bytes", c_i64), ("peak_alloc_bytes", c_i64)]
 
 
 # name -> (restype, argtypes); every entry point of include/fs3d.h

@@ -8,8 +8,10 @@
 // arrives while others run joins at the next round boundary and its ticks
 // overlap with theirs on the device.  Rounds are planned on the host ahead of
 // the GPU (tick counts are known), two at a time, with double-buffered tables.
-// A request's final tick hashes its state (fused), and its memory returns to a
-// size-class free list once that round has completed.
+// A request's final tick hashes its state (fused), and its memory returns to the
+// grid arena once that round has completed.  The HBM budget bounds the device
+// memory the arena has actually allocated (GridArena, arena.h); a request that
+// does not fit waits at the head of the queue until finished grids come back.
 //
 // Latency accounting: every launch is bracketed by HIP events, so each launch
 // duration is one tick-latency sample for the worlds in it; request latency is
@@ -25,6 +27,7 @@
 #include <deque>
 #include <vector>
 
+#include "arena.h"
 #include "fs3d_batch.h"
 
 using namespace fs3d;
@@ -45,7 +48,6 @@ struct Request {
   double t_submit = 0, t_admit = 0, t_done = 0;
   u8 *g = nullptr;
   size_t bytes = 0;
-  int cls = -1;
   int slot = -1;
 };
 
@@ -69,16 +71,20 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Size classes: 4 steps per power of two from 4 KiB (waste <= 25%).
-size_t class_bytes(int c) {
-  const int p = c / 4, q = c % 4;
-  return ((size_t)4096 << p) / 4 * (4 + q);
+// A request's grid bytes in the arena: the voxels plus the tick's guard bytes.
+size_t grid_bytes(const Request &r) { return GridArena::round_up((size_t)r.nx * r.ny * r.nz + kGuardBytes); }
+
+// Device chunks of the grid arena.  A failed hipMalloc returns nullptr and
+// leaves no sticky error behind.
+void *chunk_alloc(size_t bytes, void *) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
 }
-int class_of(size_t bytes) {
-  int c = 0;
-  while (class_bytes(c) < bytes) c++;
-  return c;
-}
+void chunk_free(void *p, void *) { (void)hipFree(p); }
 
 double percentile(std::vector<double> v, double q) {
   if (v.empty()) return 0.0;
@@ -97,11 +103,7 @@ struct fs3d_server {
   int seg = 8;
   hipStream_t s_tick = nullptr, s_fill = nullptr;
   hipEvent_t fill_ev = nullptr;
-  // arena
-  std::vector<u8 *> chunks;
-  size_t chunk_bytes = 0, chunk_used = 0;
-  std::vector<std::vector<u8 *>> free_lists;
-  size_t used = 0, peak = 0;
+  GridArena arena;  // resident grids, budget on allocated bytes
   // slots and hashes
   u64 *d_hashes = nullptr;
   u32 *d_chg = nullptr;  // settled-world flags, four per slot (tick mod 4)
@@ -129,36 +131,9 @@ int check_server(const fs3d_server *s) {
   return FS3D_OK;
 }
 
-int alloc_grid(fs3d_server *s, Request &r) {
-  const size_t need = (size_t)r.nx * r.ny * r.nz + kGuardBytes;
-  r.cls = class_of(need);
-  r.bytes = class_bytes(r.cls);
-  if ((size_t)r.cls >= s->free_lists.size()) s->free_lists.resize(r.cls + 1);
-  auto &fl = s->free_lists[r.cls];
-  if (!fl.empty()) {
-    r.g = fl.back();
-    fl.pop_back();
-  } else {
-    if (s->chunks.empty() || s->chunk_used + r.bytes > s->chunk_bytes) {
-      const size_t sz = std::max(s->chunk_bytes, r.bytes);
-      u8 *c = nullptr;
-      FS3D_HIP(hipMalloc(&c, sz));
-      s->chunks.push_back(c);
-      s->chunk_bytes = std::max(s->chunk_bytes, sz);
-      s->chunk_used = 0;
-    }
-    r.g = s->chunks.back() + s->chunk_used;
-    s->chunk_used += r.bytes;
-  }
-  s->used += r.bytes;
-  s->peak = std::max(s->peak, s->used);
-  return FS3D_OK;
-}
-
 void free_grid(fs3d_server *s, Request &r) {
   if (!r.g) return;
-  s->free_lists[r.cls].push_back(r.g);
-  s->used -= r.bytes;
+  s->arena.release(r.g);
   r.g = nullptr;
   s->free_slots.push_back(r.slot);
   r.slot = -1;
@@ -221,21 +196,39 @@ int reap(fs3d_server *s) {
   return FS3D_OK;
 }
 
+// Admits queued requests in arrival order while a hash slot is free and the
+// arena can place their grids within the budget; stops at the first that does
+// not fit (it waits for finished grids to come back).  A request larger than
+// the whole budget, or whose chunk allocation fails while no other request
+// holds memory that could come back, is rejected.
 int admit(fs3d_server *s) {
   bool any = false;
   while (!s->queue.empty()) {
     Request &r = s->reqs[s->queue.front()];
-    const size_t need = class_bytes(class_of((size_t)r.nx * r.ny * r.nz + kGuardBytes));
+    const size_t need = grid_bytes(r);
     if (need > (size_t)s->cfg.budget_bytes) {
       r.status = kStatusRejected;
       s->n_rejected++;
       s->queue.pop_front();
       continue;
     }
-    if (s->used + need > (size_t)s->cfg.budget_bytes || s->free_slots.empty()) break;
+    if (s->free_slots.empty()) break;
+    u8 *g = nullptr;
+    const GridArena::Result res = s->arena.alloc(need, &g);
+    if (res != GridArena::kOk) {
+      const bool others = s->arena.live_blocks() > 0;
+      if (res == GridArena::kNoMemory && !others) {
+        r.status = kStatusRejected;  // nothing will come back: the device cannot hold it
+        s->n_rejected++;
+        s->queue.pop_front();
+        continue;
+      }
+      break;  // back-pressure until finished grids are released
+    }
+    r.g = g;
+    r.bytes = need;
     r.slot = s->free_slots.back();
     s->free_slots.pop_back();
-    if (int rc = alloc_grid(s, r)) return rc;
     FS3D_HIP(launch_fill_rain(r.g, r.nx, r.ny, r.nz, r.seed, r.thr, s->s_fill));
     FS3D_HIP(hipMemsetAsync(s->d_hashes + r.slot, 0, sizeof(u64), s->s_fill));
     // all four change flags set: the first ticks of a new world always run
@@ -375,7 +368,8 @@ int fs3d_server_create(const fs3d_server_config *cfg, fs3d_server **out) {
   s->dev = dev;
   s->cfg = *cfg;
   s->seg = cfg->seg ? cfg->seg : 8;
-  s->chunk_bytes = std::min<size_t>((size_t)cfg->budget_bytes, (size_t)1 << 30);
+  s->arena.init((size_t)cfg->budget_bytes, std::min<size_t>((size_t)cfg->budget_bytes, (size_t)1 << 30),
+                 chunk_alloc, chunk_free, nullptr);
   hipError_t e = hipStreamCreateWithFlags(&s->s_tick, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_fill, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fill_ev, hipEventDisableTiming);
@@ -404,7 +398,7 @@ int fs3d_server_destroy(fs3d_server *s) {
     for (hipEvent_t e : rd.ev) (void)hipEventDestroy(e);
     if (rd.done) (void)hipEventDestroy(rd.done);
   }
-  for (u8 *c : s->chunks) (void)hipFree(c);
+  s->arena.clear();
   if (s->d_hashes) (void)hipFree(s->d_hashes);
   if (s->d_chg) (void)hipFree(s->d_chg);
   if (s->fill_ev) (void)hipEventDestroy(s->fill_ev);
@@ -505,8 +499,10 @@ int fs3d_server_get_stats(const fs3d_server *s, fs3d_server_stats *out) {
   st.req_p99_ms = percentile(lat, 0.99);
   st.voxel_updates = s->voxel_updates;
   st.busy_ms = s->busy_ms;
-  st.peak_bytes = (int64_t)s->peak;
-  st.used_bytes = (int64_t)s->used;
+  st.peak_bytes = (int64_t)s->arena.peak_in_use();
+  st.used_bytes = (int64_t)s->arena.in_use();
+  st.alloc_bytes = (int64_t)s->arena.allocated();
+  st.peak_alloc_bytes = (int64_t)s->arena.peak_allocated();
   st.peak_active = s->peak_active;
   *out = st;
   return clear_error();

@@ -309,12 +309,16 @@ int fs3d_palette(float *rgba);
 /* Continuous batching of simulation requests.  No reference counterpart (the
  * reference is one interactive window, src/engine/engine.cpp:54-74).  Requests
  * (dims, rain-fill seed and threshold, tick count) are admitted in arrival order
- * while their grids fit budget_bytes (grids are rounded to size classes with at
- * most 25% slack; a request larger than the whole budget is rejected), filled on
- * the device, kept resident, and advanced in rounds of up to round_ticks ticks,
+ * while their grids fit budget_bytes, filled on the device, kept resident, and
+ * advanced in rounds of up to round_ticks ticks,
  * each tick one wide launch over every active world; requests arriving meanwhile
  * join at the next round.  A request's final hash is fused into its last tick.
- * Calls on one server must be serialised. */
+ * Memory: grids (nx*ny*nz + 256 bytes, rounded to 256) are placed in device
+ * chunks of up to 1 GiB, best fit, and coalesced when released; the budget
+ * bounds the chunk bytes actually allocated.  A request that does not fit waits
+ * at the head of the queue until finished grids come back; one larger than the
+ * whole budget, or one the device cannot hold while no other grid is resident,
+ * is rejected.  Calls on one server must be serialised. */
 typedef struct fs3d_server fs3d_server; /* opaque */
 enum { FS3D_REQ_QUEUED = 0, FS3D_REQ_ACTIVE = 1, FS3D_REQ_DONE = 2, FS3D_REQ_REJECTED = 3 };
 typedef struct fs3d_server_config {
@@ -331,9 +335,10 @@ typedef struct fs3d_server_stats {
   double req_p50_ms, req_p99_ms;                /* submission -> final hash observed */
   double voxel_updates;                         /* cell updates enqueued so far */
   double busy_ms;                               /* summed device time of completed launches */
-  int64_t peak_bytes, used_bytes;               /* admitted grid bytes: high-water mark, now */
+  int64_t peak_bytes, used_bytes;               /* grid bytes in use: high-water mark, now */
   int32_t peak_active;
   int32_t reserved;
+  int64_t alloc_bytes, peak_alloc_bytes;        /* device bytes the grid arena holds: now, high-water mark */
 } fs3d_server_stats;
 int fs3d_server_create(const fs3d_server_config *cfg, fs3d_server **out);
 int fs3d_server_destroy(fs3d_server *s); /* NULL is a no-op; waits for work in flight */
