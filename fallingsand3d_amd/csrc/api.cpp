@@ -680,9 +680,10 @@ static int save_chunk(void *ctx, size_t off, u8 *buf, size_t n) {
   return FS3D_OK;
 }
 
+// A load streams the payload into the world's idle buffer.
 static int load_chunk(void *ctx, size_t off, const u8 *buf, size_t n) {
   fs3d_world *w = static_cast<fs3d_world *>(ctx);
-  FS3D_HIP(hipMemcpyAsync(w->buf[w->cur] + off, buf, n, hipMemcpyHostToDevice, w->stream));
+  FS3D_HIP(hipMemcpyAsync(w->buf[w->cur ^ 1] + off, buf, n, hipMemcpyHostToDevice, w->stream));
   FS3D_HIP(hipStreamSynchronize(w->stream));
   return FS3D_OK;
 }
@@ -705,15 +706,32 @@ int fs3d_world_load(fs3d_world *w, const char *path) {
     return set_error(FS3D_ERR_INVALID, "%s holds a %dx%dx%d world, this one is %dx%dx%d", path, dims[0], dims[1],
                      dims[2], w->nx, w->ny, w->nz);
   FS3D_HIP(hipSetDevice(w->dev));
-  if (int rc = file_read(path, nullptr, w->n, load_chunk, w)) return rc;
-  w->tick = tick;
-  if (int rc = mark_all(w)) return rc;
+  // The payload goes to the idle buffer and becomes the world's state only once
+  // its hash matches the stored one; on any failure the grid and tick stay as
+  // they were.  The idle buffer holds the previous tick's output, which skipped
+  // segments rely on, so every change bit is set either way (mark_all): the
+  // next two ticks evaluate every segment and rewrite both buffers.
+  int rc = file_read(path, nullptr, w->n, load_chunk, w);
   u64 h = 0;
-  if (int rc = fs3d_world_hash(w, &h)) return rc;
-  if (h != want)
-    return set_error(FS3D_ERR_STATE, "%s: grid hash %016llx does not match the stored %016llx", path,
+  if (!rc) {
+    u8 *idle = w->buf[w->cur ^ 1];
+    hipError_t e = launch_hash(idle, w->n, w->d_parts, w->stream);
+    if (e == hipSuccess) e = launch_hash_finalize(w->d_parts, w->d_small + 1, w->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w->h_pinned, w->d_small + 1, sizeof(u64), hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) rc = set_error(FS3D_ERR_HIP, "world load hash failed: %s", hipGetErrorString(e));
+    h = w->h_pinned[0];
+    if (!rc && h != want)
+      rc = set_error(FS3D_ERR_STATE, "%s: grid hash %016llx does not match the stored %016llx", path,
                      (unsigned long long)h, (unsigned long long)want);
-  return clear_error();
+  }
+  if (!rc) {
+    w->cur ^= 1;
+    w->tick = tick;
+  }
+  if (int rc2 = mark_all(w)) return rc ? rc : rc2;
+  if (hipStreamSynchronize(w->stream) != hipSuccess && !rc) rc = set_error(FS3D_ERR_HIP, "world load sync failed");
+  return rc ? rc : clear_error();
 }
 
 int fs3d_world_fill_rain(fs3d_world *w, uint64_t seed, uint64_t thr) {

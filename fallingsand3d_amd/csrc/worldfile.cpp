@@ -104,6 +104,12 @@ int file_info(const char *path, int32_t dims[3], u64 *tick, u64 *state_hash, u64
     if (h.dims[a] < 1) return set_error(FS3D_ERR_INVALID, "%s: bad dims", path);
   if (h.payload != (u64)h.dims[0] * h.dims[1] * h.dims[2])
     return set_error(FS3D_ERR_INVALID, "%s: payload size does not match dims", path);
+  // the file holds exactly the header and the payload
+  if (fseeko(f.f, 0, SEEK_END) != 0) return set_error(FS3D_ERR_INVALID, "cannot seek %s", path);
+  const off_t len = ftello(f.f);
+  if (len < 0 || (u64)len != kHeaderBytes + h.payload)
+    return set_error(FS3D_ERR_INVALID, "%s: %lld bytes, header and payload need %llu", path, (long long)len,
+                     (unsigned long long)(kHeaderBytes + h.payload));
   if (dims) std::memcpy(dims, h.dims, sizeof h.dims);
   if (tick) *tick = h.tick;
   if (state_hash) *state_hash = h.state_hash;

@@ -54,6 +54,13 @@ def test_rejects_bad_files(tmp_path):
         p.write_bytes(bytes(data))
         with pytest.raises(F.Fs3dError):
             F.file_read(p)
+    # the header alone must already refuse a file of the wrong length
+    for name, data in {"truncated": raw[:-1], "extended": raw + b"\0"}.items():
+        p = tmp_path / f"info_{name}.fs3d"
+        p.write_bytes(bytes(data))
+        with pytest.raises(F.Fs3dError) as ei:
+            F.file_info(p)
+        assert "bytes" in str(ei.value)
     with pytest.raises(F.Fs3dError):
         F.file_info(tmp_path / "missing.fs3d")
     with pytest.raises(F.Fs3dError):
@@ -105,3 +112,38 @@ def test_load_errors(gpu, tmp_path):
     with gpu.World((20, 10, 8)) as v:
         with pytest.raises(gpu.Fs3dError):
             v.load(bad)
+
+
+@pytest.mark.gpu
+def test_failed_load_leaves_the_world_as_it_was(gpu, O, tmp_path):
+    # a corrupt payload is read into the idle buffer before its hash is checked;
+    # the world keeps its grid and tick, and its next ticks (which would skip
+    # settled segments on the idle buffer's old contents) stay exact
+    announce("world load: corrupt and truncated files leave grid, tick and later ticks unchanged")
+    dims = (40, 24, 20)
+    ck = tmp_path / "ck.fs3d"
+    with gpu.World(dims) as src:
+        src.fill_rain(8, gpu.threshold(1, 3))
+        src.step(5)
+        src.save(ck)
+    raw = bytearray(ck.read_bytes())
+    bad = tmp_path / "bad.fs3d"
+    raw[64 + 40 * 24 * 11 + 7] ^= 0x01
+    bad.write_bytes(bytes(raw))
+    short = tmp_path / "short.fs3d"
+    short.write_bytes(bytes(raw[:-100]))
+    with gpu.World(dims) as w, gpu.World(dims) as ref:
+        for v in (w, ref):
+            v.fill_boxes(3, side_min=4, side_max=12)
+            v.step(9)
+        h0, t0 = w.hash(), w.tick_count
+        for f in (bad, short):
+            with pytest.raises(gpu.Fs3dError):
+                w.load(f)
+            assert w.hash() == h0 and w.tick_count == t0
+        assert [int(x) for x in w.run(30, 5)] == [int(x) for x in ref.run(30, 5)]
+        assert (w.download() == ref.download()).all()
+        w.load(ck)  # a good file still loads after the failures
+        g = O.scene_rain(dims, 8, O.threshold(1, 3))
+        g, _ = O.run(g, 0, 5)
+        assert w.tick_count == 5 and (w.download() == g).all()
