@@ -87,8 +87,57 @@ def test_viewer_without_gpu_fails_loudly(tmp_path):
     assert r.returncode == 1 and "ERROR: " in r.stderr
 
 
+def read_ppm(path):
+    data = open(path, "rb").read()
+    head, W, H = b"", 0, 0
+    parts = data.split(b"\n", 3)
+    assert parts[0] == b"P6" and parts[2] == b"255"
+    W, H = map(int, parts[1].split())
+    return np.frombuffer(parts[3], np.uint8).reshape(H, W, 3)
+
+
+def quantize(rgb):
+    """The viewer's PPM rounding: clamp to [0, 1], then (uint8)(v * 255 + 0.5) in fp32."""
+    v = np.clip(rgb.astype(np.float32), np.float32(0), np.float32(1))
+    return (v * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+
+
+def check_frames_against_oracle(out, dims, tpf, O):
+    from fallingsand3d_amd import render as R
+    W_KEY, RIGHT_KEY = 1, 128
+    pal = np.array(json.load(open(os.path.join(ROOT, "tests", "golden", "palette.json")))["rgba"], np.float32)
+    # (drawn frame, ticks before its render, keys held on each drawn frame up to it)
+    held = [W_KEY] * 5 + [RIGHT_KEY] * 4 + [0]
+    g0, _ = O.scene_boxes(dims, 7, side_min=8, side_max=32)
+    cases = [(0, 1 * tpf), (8, 9 * tpf), (9, 13 * tpf)]
+    for drawn, nticks in cases:
+        announce(f"viewer frame {drawn} vs oracle render ({nticks} ticks)")
+        if drawn == 9:  # the edit lands before the last frame's ticks
+            g, _ = O.run(g0, 0, nticks - tpf, use_lut=True)
+            g[60, 30, 20] = 2
+            g, _ = O.run(g, nticks - tpf, tpf, use_lut=True)
+        else:
+            g, _ = O.run(g0, 0, nticks, use_lut=True)
+        pos, rot = (0.0, 0.0, -5.0), (0.0, 0.0, 0.0)
+        for keys in held[:drawn + 1]:
+            pos, rot = R.camera_integrate(pos, rot, keys)
+        img = read_ppm(os.path.join(out, f"frame_{drawn:06d}.ppm"))
+        H, W = img.shape[:2]
+        cam = R.key_camera(dims, pos, rot, W / H)
+        ocam = O.camera_from(list(cam.origin), list(cam.basis), list(cam.light), cam.aspect)
+        rgb, hit, _ = O.render_dda(g, ocam, pal, W, H)
+        want = quantize(rgb)
+        diff = np.abs(img.astype(np.int16) - want.astype(np.int16))
+        # RGB within 1e-5 of the oracle can round to the neighbouring byte only
+        # where v * 255 + 0.5 lies within 0.003 of an integer
+        assert diff.max() <= 1, (drawn, int(diff.max()))
+        near = np.abs((np.clip(rgb, 0, 1) * 255 + 0.5) % 1.0 - 0.5) > 0.497
+        assert not (diff.astype(bool) & ~near).any(), drawn
+        assert (hit != R.MISS).mean() > 0.02  # the grid is in view (5 half-extents away)
+
+
 @pytest.mark.gpu
-def test_viewer_session_matches_plain_world(gpu, tmp_path):
+def test_viewer_session_matches_plain_world(gpu, O, tmp_path):
     exe = build_viewer(tmp_path)
     script = tmp_path / "session.txt"
     script.write_text(
@@ -119,6 +168,9 @@ def test_viewer_session_matches_plain_world(gpu, tmp_path):
     img = open(out / "frame_000000.ppm", "rb").read()
     assert img.startswith(b"P6\n64 40\n255\n") and len(img) == len(b"P6\n64 40\n255\n") + 64 * 40 * 3
     assert max(img[14:]) > 0  # the grid is in view
+    # frames 0, 8 and 9 against the oracle's render of the oracle's grid at that
+    # tick, under the camera the held keys give, after the PPM's 8-bit rounding
+    check_frames_against_oracle(out, dims, tpf, O)
     # the same ticks and edit on a plain world give the same state
     with gpu.World(dims) as w:
         w.fill_boxes(7, side_min=8, side_max=32)
