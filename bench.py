@@ -2,28 +2,38 @@
 
 Default run (one JSON line on stdout):
   value    the 512^3 box scene (20% sand, 10% fluid, 5% static boxes over a floor),
-           K ticks in one captured HIP graph with a fused state hash every 50
-           ticks, after W warm-up ticks, with the library's default tick (settled
-           segments skipped, results bit-identical to the dense tick).  A "step" is
-           one tick of the whole world; value counts every voxel of every tick.
+           K ticks in captured HIP graphs with a fused state hash every 50 ticks
+           and always on the last timed tick, after W warm-up ticks, with the
+           library's default tick (settled segments skipped, results bit-identical
+           to the dense tick).  A "step" is one tick of the whole world; value
+           counts every voxel of every tick.
+  parity   every hash of the timed region against the oracle's per-tick hashes of
+           the same scene (tests/golden/boxes512_hashes.txt).
   roofline the timed region: algorithmic bytes actually moved (2 per swept
            voxel-update, 1 per voxel only read on hash ticks; counted by replaying
-           the same ticks with sweep counters) / device time of the timed graph
-           (HIP events on the world's stream), against 8.0 TB/s; "dense" holds the
-           dense tick kernel on the same world (2 bytes per voxel-update).
+           the same ticks with sweep counters) / device time of the timed graphs
+           (HIP events on the world's stream), against 8.0 TB/s.  "traffic" is
+           measured for the same window: this script runs the window again in two
+           child processes under rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE),
+           scaled by the counters of a copy of known size in the same process.
+           "dense" holds the dense tick kernel on the same world; "hbm2048" the
+           dense tick of the 2048^3 pillar scene, whose buffers (2 x 8 GiB) do not
+           fit the 256 MB Infinity Cache: the HBM-bound figure.
   extra    pillars2048: the 2048^3 pillar scene, skipping, dense and dense fused
-           in pairs (the dense single-tick kernel is the HBM-bound figure: the
-           512^3 pair of buffers nearly fits the 256 MB Infinity Cache);
-           pillars2048_render: that scene with a
-           1920x1080 voxel render every 10 ticks; copy: a plain device copy of the
-           same bytes, the practical bandwidth ceiling; batched: 4096 worlds against
-           the one-launch-per-world loop; serving: the continuous-batching server
-           under a Poisson load.
+           in pairs; pillars2048_render: that scene with a 1920x1080 voxel render
+           every 10 ticks; copy: a plain device copy of the same bytes, the
+           practical bandwidth ceiling; batched: 4096 worlds against the
+           one-launch-per-world loop; serving: the continuous-batching server
+           under a Poisson load, saturated, and against one world per request.
+           Each workload's hashes (and frame digests) are compared with the
+           oracle's fixtures in tests/golden/.
   cpu_baseline  the oracle's tick on this host, one thread, a bounded sample; its
            "openmp" field times the same tick over the host threads the run may use.
 
 --only {boxes512,pillars2048,copy,batched,serving,render2048} runs one workload
-(for profiler runs) and prints its own JSON line.
+(for profiler runs) and prints its own JSON line; --only boxes512-window and
+--only dense2048-window run just a timed window (and a copy of known size), for
+the counter runs.
 """
 import argparse
 import json
@@ -36,11 +46,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 BYTES_PER_UPDATE = 2
-# Bytes per tick of the timed region from rocprofv3 counter runs of this same
-# command (profiles/tick_kernel_boxes512.md): FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB
-# per dispatch (factors from a copy of known size), averaged over its skipping
-# and hashing ticks.
-PMC_TRAFFIC_BOXES512 = 171.2e6
+HASH_EVERY = 50
+COPY_CAL_BYTES = 1 << 30  # the known-size copy that scales the byte counters
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 METRIC = "Gvoxel-updates/s + %HBM roofline at 512^3/2048^3; batched vs plain; p99 tick ms"
 
 
@@ -53,25 +61,61 @@ def roofline(vox, kern_ms):
     return ach, ach / HBM_PEAK
 
 
+def golden_rows(name):
+    """Rows of integers of an oracle fixture under tests/golden ('#' lines are
+    comments), or None when the file is absent."""
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return [[int(v) for v in line.split()] for line in f if line.strip() and not line.startswith("#")]
+
+
+def hash_segments(steps, every=HASH_EVERY):
+    """The timed ticks as graph runs (nticks, hash_every): whole groups of `every`
+    ticks, hashed at each group's end, then the remainder as one run hashed at
+    its last tick, so the last timed tick is always hashed."""
+    q, r = divmod(steps, every)
+    return ([(q * every, every)] if q else []) + ([(r, r)] if r else [])
+
+
+def timed_boxes_window(w, warmup, steps):
+    """Warm-up ticks, then the timed graph runs of hash_segments(steps).  Both
+    graphs are captured beforehand: the first run has an even tick count, so the
+    second starts from the same buffer it was captured on.  Returns (wall s,
+    device ms per tick, hashes, absolute tick of each hash)."""
+    if warmup:
+        w.run(warmup, 0)
+    segs = hash_segments(steps)
+    for nt, he in segs:
+        w.prepare_run(nt, he)
+    w.sync()
+    hashes, dev = [], 0.0
+    t0 = time.perf_counter()
+    for nt, he in segs:
+        hashes += [int(x) for x in w.run(nt, he)]
+        dev += w.last_run_ms_per_tick() * nt
+    wall = time.perf_counter() - t0
+    ticks, t = [], warmup
+    for nt, he in segs:
+        ticks += [t + he * (i + 1) for i in range(nt // he)]
+        t += nt
+    return wall, dev / steps, hashes, ticks
+
+
 def bench_boxes512(F, steps, warmup, kernel_ticks=200):
     """The 512^3 box scene with the library's default tick (settled segments are
-    skipped): warm-up ticks, then the timed graph of `steps` ticks with a fused
-    hash every 50.  The same tick sequence is replayed with sweep counters on to
-    get the algorithmic bytes of the timed region (skipping is deterministic), and
-    a dense world runs the plain tick kernel for the dense roofline."""
+    skipped): warm-up ticks, then the timed graphs of `steps` ticks with a fused
+    hash every 50 and on the last tick.  The same tick sequence is replayed with
+    sweep counters on to get the algorithmic bytes of the timed region (skipping
+    is deterministic), and a dense world runs the plain tick kernel for the dense
+    roofline."""
     n = 512
     with F.World((n, n, n)) as w:
         t0 = time.perf_counter()
         w.fill_boxes(1)
         log(f"[bench] boxes512: scene generated in {time.perf_counter() - t0:.2f}s")
-        if warmup:
-            w.run(warmup, 0)
-        w.prepare_run(steps, 50)  # graph capture is set-up, not a tick
-        w.sync()
-        t0 = time.perf_counter()
-        hashes = w.run(steps, 50)
-        wall = time.perf_counter() - t0
-        dev_ms = w.last_run_ms_per_tick()
+        wall, dev_ms, hashes, hticks = timed_boxes_window(w, warmup, steps)
         log(f"[bench] boxes512: {steps} ticks in {wall * 1e3:.2f} ms ({dev_ms * 1e3:.2f} us/tick on the device)")
         # replay with counters
         w.fill_boxes(1)
@@ -79,9 +123,11 @@ def bench_boxes512(F, steps, warmup, kernel_ticks=200):
         if warmup:
             w.run(warmup, 0)
         w.sweep_stats(True)
-        h2 = w.run(steps, 50)
+        h2 = []
+        for nt, he in hash_segments(steps):
+            h2 += [int(x) for x in w.run(nt, he)]
         st = w.sweep_stats(False)
-        if [int(x) for x in h2] != [int(x) for x in hashes]:
+        if h2 != hashes:
             raise SystemExit("boxes512: counter replay diverged from the timed run")
         mode, seg = w.skip, w.segment
     with F.World((n, n, n), flags=F.DENSE) as d:
@@ -89,8 +135,104 @@ def bench_boxes512(F, steps, warmup, kernel_ticks=200):
         d.run(kernel_ticks, 0)
         d.run(kernel_ticks, 0)
         kern_ms = d.last_run_ms_per_tick()
-    return dict(vox=n ** 3, wall=wall, dev_ms=dev_ms, kern_ms=kern_ms, hashes=[int(h) for h in hashes], seg=seg,
+    return dict(vox=n ** 3, wall=wall, dev_ms=dev_ms, kern_ms=kern_ms, hashes=hashes, hash_ticks=hticks, seg=seg,
                 swept=st["swept"], read_only=st["read_only"], violations=st["violations"], mode=mode)
+
+
+def boxes512_parity(hashes, ticks):
+    """Every hash of the timed region against the oracle's per-tick hashes."""
+    fx = golden_rows("boxes512_hashes.txt")
+    if fx is None or not ticks or max(ticks) >= len(fx):
+        return {"checked": 0, "match": None, "fixture": "tests/golden/boxes512_hashes.txt",
+                "note": "fixture absent or shorter than warm-up + steps"}
+    bad = [t for h, t in zip(hashes, ticks) if fx[t][0] != h]
+    return {"checked": len(ticks), "match": not bad, "ticks": [ticks[0], ticks[-1]],
+            "fixture": "tests/golden/boxes512_hashes.txt (oracle, hash after every tick)",
+            **({"mismatched_ticks": bad[:8]} if bad else {})}
+
+
+def window_main(F, mode, steps, warmup):
+    """A timed window alone, then a copy of COPY_CAL_BYTES: what the counter runs
+    profile (one dispatch per tick of the window's tick kernel)."""
+    if mode == "boxes512-window":
+        with F.World((512, 512, 512)) as w:
+            w.fill_boxes(1)
+            _, _, hashes, ticks = timed_boxes_window(w, warmup, steps)
+        out = {"workload": mode, "hash": hashes[-1], "tick": ticks[-1]}
+    else:
+        with F.World((2048, 2048, 2048), flags=F.DENSE) as w:
+            w.fuse = False
+            w.fill_pillars(2)
+            w.run(steps, 0)
+            out = {"workload": mode, "hash": int(w.hash()), "tick": steps}
+    F.copy_probe(COPY_CAL_BYTES, 3)
+    print(json.dumps(out), flush=True)
+
+
+def pmc_traffic(mode, steps, warmup, kernel_sub, last_n):
+    """HBM-side bytes per launch of the last `last_n` dispatches whose kernel
+    name contains `kernel_sub`, from two child runs of `--only mode` under
+    rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  Each
+    counter is scaled by what it reads for the copy of COPY_CAL_BYTES in the
+    same child.  Runs before this process touches the GPU.  Returns a dict, or
+    {"error": ...}."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return {"error": "rocprofv3 not found"}
+    base = tempfile.mkdtemp(prefix="fs3d_pmc_")
+    vals = {}
+    try:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(base, ctr)
+            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+                   os.path.abspath(__file__), "--only", mode, "--steps", str(steps), "--warmup", str(warmup),
+                   "--no-cpu"]
+            log(f"[bench] traffic: {ctr} counter run of --only {mode} --steps {steps} --warmup {warmup}")
+            t0 = time.perf_counter()
+            p = subprocess.Popen(cmd, cwd=base, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                 start_new_session=True)
+            try:
+                text, _ = p.communicate(timeout=240)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return {"error": f"{ctr} counter run timed out"}
+            done = f'"workload": "{mode}"' in text
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            # rocprofv3 may end with SIGSEGV after the program finished and its results are whole
+            if not done or not files or p.returncode not in (0, 139, -11):
+                tail = " | ".join(text.strip().splitlines()[-3:])
+                return {"error": f"{ctr} counter run failed (rc {p.returncode}): {tail[-300:]}"}
+            per = {}
+            with open(files[0]) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != ctr:
+                        continue
+                    did = int(r["Dispatch_Id"])
+                    name, v = per.get(did, (r["Kernel_Name"], 0.0))
+                    per[did] = (name, v + float(r["Counter_Value"]))
+            disp = [per[k] for k in sorted(per)]
+            ticks = [v for name, v in disp if kernel_sub in name][-last_n:]
+            copies = [v for name, v in disp if "copy_kernel" in name]
+            if len(ticks) != last_n or not copies:
+                return {"error": f"{ctr}: {len(ticks)} of {last_n} tick dispatches, {len(copies)} copies found"}
+            vals[ctr] = (sum(ticks) / len(ticks), sum(copies) / len(copies))
+            log(f"[bench] traffic: {ctr} {vals[ctr][0]:.1f} KiB per dispatch ({time.perf_counter() - t0:.1f} s)")
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    f_read = COPY_CAL_BYTES / (vals["FETCH_SIZE"][1] * 1024.0)
+    f_write = COPY_CAL_BYTES / (vals["WRITE_SIZE"][1] * 1024.0)
+    rd = vals["FETCH_SIZE"][0] * 1024.0 * f_read
+    wr = vals["WRITE_SIZE"][0] * 1024.0 * f_write
+    return {"bytes": rd + wr, "read": rd, "written": wr, "dispatches": last_n,
+            "scale": {"FETCH_SIZE": round(f_read, 4), "WRITE_SIZE": round(f_write, 4)}}
 
 
 def bench_pillars2048(F, ticks):
@@ -143,13 +285,28 @@ def orbit_angles(seed, n):
     return out
 
 
+def golden_frames():
+    path = os.path.join(GOLDEN, "pillars2048_frames.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["frames"]
+
+
 def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
     """The 2048^3 collapsing-column scene: `ticks` ticks in graphs of `every`,
-    a W x H voxel render after each, from a seeded orbit camera."""
+    a W x H voxel render after each, from a seeded orbit camera.  After each
+    frame (outside the timed loop) the state hash and the SHA-256 of the hit and
+    face maps are compared with the oracle's (tests/golden/pillars2048_frames.json)."""
+    import hashlib
+
+    import numpy as np
+
     from fallingsand3d_amd import render as R
     n = 2048
-    frames, tick_ms, render_ms, steps = [], [], [], 0
+    frames, tick_ms, render_ms, steps, checks = [], [], [], 0, []
     angles = orbit_angles(seed, ticks // every)
+    fx = golden_frames()
     with F.World((n, n, n)) as w:
         w.fill_pillars(seed)
         # a first render turns on occupancy tracking (this grid is above 2^29
@@ -166,14 +323,22 @@ def bench_pillars_render(F, ticks=200, every=10, W=1920, H=1080, seed=2):
             rgb, hit, face = R.render(w, cam, W, H)
             render_ms.append(R.last_render_ms(w))
             frames.append(float((hit != R.MISS).mean()))
-            t1 = time.perf_counter()
+            t1 = time.perf_counter()  # counting and checking are not part of the loop's wall time
             steps += voxel_steps(cam, w.dims, hit, W, H)
-            t0 += time.perf_counter() - t1  # host-side counting is not part of the loop's wall time
+            got = dict(tick=(k + 1) * every, hash=int(w.hash()),
+                       hit_sha256=hashlib.sha256(np.ascontiguousarray(hit, "<u8").tobytes()).hexdigest(),
+                       face_sha256=hashlib.sha256(np.ascontiguousarray(face).tobytes()).hexdigest())
+            if fx is not None and k < len(fx):
+                checks.append(all(fx[k][key] == got[key] for key in got))
+            t0 += time.perf_counter() - t1
         wall = time.perf_counter() - t0
         h = w.hash()
     return dict(wall=wall, tick_ms=sum(tick_ms) / len(tick_ms), render_ms=sum(render_ms) / len(render_ms),
                 render_ms_max=max(render_ms), hit_frac=sum(frames) / len(frames), frames=len(frames), hash=int(h),
-                vox=n ** 3, ticks=ticks, rays=W * H, voxel_steps=steps / len(frames))
+                vox=n ** 3, ticks=ticks, rays=W * H, voxel_steps=steps / len(frames),
+                parity={"frames_checked": len(checks), "match": all(checks) if checks else None,
+                        "what": "state hash after every 10th tick, SHA-256 of each frame's hit and face maps, "
+                                "against the oracle (tests/golden/pillars2048_frames.json)"})
 
 
 def voxel_steps(cam, dims, hit, W, H):
@@ -223,10 +388,18 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
     density in [0.05, 0.5): all worlds step `ticks` ticks in one persistent
     launch (shape chosen by the autotuner) against the plain loop of one launch
     per world per tick.  The plain loop is timed on its first `plain_ticks`
-    ticks (2 x 4096 x 20 launches would not change its per-tick cost) and its
-    hashes are compared with the persistent kernel's over the same ticks."""
+    ticks (2 x 4096 x 20 launches would not change its per-tick cost).  The
+    hashes after tick 20 (plain and persistent) and after tick 500 (persistent,
+    with and without settled-world skipping) are compared with the oracle's
+    (tests/golden/batch4096_hashes.txt)."""
+    import numpy as np
     dims, thr, seeds = F.batch_specs(nworlds, seed, 16, 128)
     vox = int(dims.astype("int64").prod(1).sum())
+    fx = golden_rows("batch4096_hashes.txt")
+    fx_ok = fx is not None and len(fx) == nworlds and all(tuple(r[:3]) == tuple(int(v) for v in d)
+                                                           for r, d in zip(fx, dims))
+    want20 = np.array([r[3] for r in fx], np.uint64) if fx_ok else None
+    want_end = np.array([r[4] for r in fx], np.uint64) if fx_ok and ticks == 500 else None
     with F.Batch(dims) as b:
         b.fill_rain(seeds, thr)
         tune = b.autotune(trial_ticks=10)
@@ -238,7 +411,7 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
             h_plain = p.run(plain_ticks, plain=True)
             plain_wall = time.perf_counter() - t0
         exact = bool((h_sample == h_plain).all())
-        log(f"[bench] batched: plain loop {plain_ticks} ticks in {plain_wall:.3f}s, bit-exact={exact}")
+        log(f"[bench] batched: plain loop {plain_ticks} ticks in {plain_wall:.3f}s, plain = persistent: {exact}")
         b.tick_count = 0
         b.fill_rain(seeds, thr)
         b.run(ticks)  # graph capture + warm caches
@@ -265,6 +438,12 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
         digest = 0
         for v in h:
             digest = (digest + int(v)) % (1 << 64)
+        parity = {"fixture": "tests/golden/batch4096_hashes.txt (oracle)", "match": None}
+        if want20 is not None:
+            m20 = bool((h_plain == want20).all() and (h_sample == want20).all())
+            m_end = bool((h == want_end).all() and (h_all == want_end).all()) if want_end is not None else None
+            parity = {**parity, "tick20_plain_and_persistent": m20, f"tick{ticks}_persistent": m_end,
+                      "worlds": nworlds, "match": m20 and m_end is not False}
         plain_per_tick = plain_wall / plain_ticks
         return dict(nworlds=nworlds, vox=vox, ticks=ticks, wall=wall, dev_ms=dev_ms,
                     gvox_per_s=vox * ticks / wall / 1e9, plain_ms_per_tick=plain_per_tick * 1e3,
@@ -273,34 +452,66 @@ def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
                     all_ticks_wall=wall_all, all_ticks_gvox_per_s=vox * ticks / wall_all / 1e9,
                     all_ticks_speedup=(plain_per_tick * ticks) / wall_all,
                     tuned={"mode": {0: "persistent", 2: "wide"}.get(tune["mode"], tune["mode"]), "seg": tune["seg"],
-                           "threads": tune["threads"], "groups": tune["groups"]}, hash_digest=digest)
+                           "threads": tune["threads"], "groups": tune["groups"]}, hash_digest=digest, parity=parity)
 
 
-def bench_serving(F, seconds=3.0, lam=2000.0, seed=5, round_ticks=8, check=8):
+def bench_serving(F, seconds=3.0, lam=2000.0, seed=5, round_ticks=8, plain_every=10):
     """Poisson stream of requests at `lam` per second for `seconds` seconds of
     arrivals (edges log-uniform 32-256, ticks uniform 10-500, rain fill with
     density U(0.05, 0.5)) through the continuous-batching server under a
-    200 GiB budget, replayed in real time; `check` requests spread over the
-    stream are re-run alone and their final hashes compared."""
+    200 GiB budget, replayed in real time; then the same requests all at once
+    (the server's capacity); then the plain shape of one world per request (its
+    own grid, fill, one tick launch per tick, hash), timed on every
+    `plain_every`-th request.  Every final hash is compared with the oracle's
+    (tests/golden/serve6000_hashes.txt); the plain sample's with the served ones."""
+    import numpy as np
     n = int(seconds * lam)
     dims, thr, seeds, ticks, arr = F.serve_specs(n, seed, 32, 256, 10, 500, lam)
-    with F.Server(budget_bytes=200 << 30, round_ticks=round_ticks, max_active=1 << 16) as s:
+    cells = dims.astype(np.int64).prod(1)
+    budget = 200 << 30
+    with F.Server(budget_bytes=budget, round_ticks=round_ticks, max_active=1 << 16) as s:
         t0 = time.perf_counter()
         ids = s.replay(dims, seeds, thr, ticks, arr)
         wall = time.perf_counter() - t0
         st = s.stats()
         res = [s.result(int(i)) for i in ids]
-    exact = True
-    for i in range(0, n, max(1, n // check)):
+    log(f"[bench] serving: {n} requests in {wall:.2f}s, tick p50 {st['tick_p50_ms']:.3f} ms p99 "
+        f"{st['tick_p99_ms']:.3f} ms, request p50 {st['req_p50_ms']:.1f} ms p99 {st['req_p99_ms']:.1f} ms")
+    with F.Server(budget_bytes=budget, round_ticks=round_ticks, max_active=1 << 16) as s:
+        t0 = time.perf_counter()
+        ids2 = s.replay(dims, seeds, thr, ticks, np.zeros(n))
+        wall_sat = time.perf_counter() - t0
+        st_sat = s.stats()
+        res_sat = [s.result(int(i)) for i in ids2]
+    log(f"[bench] serving: all {n} requests at once in {wall_sat:.2f}s "
+        f"({st_sat['voxel_updates'] / wall_sat / 1e9:.1f} Gvox/s)")
+    sample = list(range(0, n, plain_every))
+    plain_hash = {}
+    t0 = time.perf_counter()
+    for i in sample:
         with F.World(tuple(int(v) for v in dims[i])) as w:
             w.fill_rain(int(seeds[i]), int(thr[i]))
             w.step(int(ticks[i]))
-            exact = exact and (w.hash() == res[i][1]) and res[i][0] == 2
-    log(f"[bench] serving: {n} requests in {wall:.2f}s, tick p50 {st['tick_p50_ms']:.3f} ms p99 "
-        f"{st['tick_p99_ms']:.3f} ms, request p50 {st['req_p50_ms']:.1f} ms p99 {st['req_p99_ms']:.1f} ms, "
-        f"bit-exact={exact}")
-    return dict(n=n, wall=wall, stats=st, exact=exact,
-                gvox_per_s=st["voxel_updates"] / wall / 1e9, busy_frac=st["busy_ms"] / (wall * 1e3))
+            plain_hash[i] = w.hash()
+    plain_wall = time.perf_counter() - t0
+    plain_vox = float((cells[sample] * ticks[sample]).sum())
+    plain_rate = plain_vox / plain_wall
+    served_ok = all(r[0] == 2 for r in res) and all(r[0] == 2 for r in res_sat)
+    same = served_ok and all(res[i][1] == res_sat[i][1] for i in range(n)) and \
+        all(plain_hash[i] == res[i][1] for i in sample)
+    fx = golden_rows("serve6000_hashes.txt")
+    parity = {"fixture": "tests/golden/serve6000_hashes.txt (oracle)", "match": None}
+    if fx is not None and len(fx) >= n and all(tuple(fx[i][:4]) == (*(int(v) for v in dims[i]), int(ticks[i]))
+                                               for i in range(n)):
+        parity = {**parity, "requests": n, "match": served_ok and all(res[i][1] == fx[i][4] for i in range(n))}
+    log(f"[bench] serving: plain loop on {len(sample)} requests in {plain_wall:.2f}s "
+        f"({plain_rate / 1e9:.1f} Gvox/s); served = saturated = plain: {same}; oracle: {parity['match']}")
+    rate = st["voxel_updates"] / wall
+    rate_sat = st_sat["voxel_updates"] / wall_sat
+    return dict(n=n, wall=wall, stats=st, exact=same, gvox_per_s=rate / 1e9, busy_frac=st["busy_ms"] / (wall * 1e3),
+                saturated=dict(wall=wall_sat, gvox_per_s=rate_sat / 1e9, stats=st_sat),
+                plain=dict(requests=len(sample), wall=plain_wall, gvox_per_s=plain_rate / 1e9),
+                speedup_vs_plain=rate / plain_rate, saturated_speedup_vs_plain=rate_sat / plain_rate, parity=parity)
 
 
 def cpu_baseline(seconds=12.0, omp_seconds=6.0):
@@ -343,25 +554,41 @@ def cpu_baseline(seconds=12.0, omp_seconds=6.0):
                                                 f"{threads} threads, {dt3:.1f} s (sampled)"))
 
 
+DENSE2048_WINDOW = 10  # dense 2048^3 ticks of the counter run for roofline.hbm2048
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving", "render2048"],
-                    default=None)
+    ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving", "render2048",
+                                       "boxes512-window", "dense2048-window"], default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs of the timed window")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every world lives on one device and nothing here shards "
               "across GPUs (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        raise SystemExit("--steps >= 1 and --warmup >= 0")
+
+    # the counter runs are child processes started before this one touches the GPU
+    traffic = traffic2048 = None
+    if args.only in (None, "boxes512") and not args.no_traffic:
+        traffic = pmc_traffic("boxes512-window", args.steps, args.warmup, "tick_kernel<", args.steps)
+        if args.only is None:
+            traffic2048 = pmc_traffic("dense2048-window", DENSE2048_WINDOW, 0, "tick_kernel<", DENSE2048_WINDOW)
 
     import fallingsand3d_amd as F
     if F.device_count() < 1:
         raise SystemExit("no GPU visible")
     F.init(0)
 
+    if args.only in ("boxes512-window", "dense2048-window"):
+        window_main(F, args.only, args.steps, args.warmup)
+        return
     if args.only == "pillars2048":
         p = bench_pillars2048(F, args.steps)
         print(json.dumps({"workload": "pillars2048", "dense_kernel_ms": p["dense"]["kern_ms"],
@@ -396,6 +623,26 @@ def main():
     frac = ach / HBM_PEAK
     ach_d, frac_d = roofline(vox, r["kern_ms"])
     sched = {1: "auto", 2: "listed", 3: "waves"}.get(r["mode"], str(r["mode"]))
+    alg_per_tick = alg_bytes / args.steps
+    roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(frac, 4), "traffic": None,
+            "kernel": f"tick_kernel<{r['seg']},*,skip> ({sched} schedule), every tick of the timed region",
+            "ms_per_tick_device": round(r["dev_ms"], 5), "alg_bytes_per_tick": int(alg_per_tick),
+            "swept_fraction": round(r["swept"] / (vox * args.steps), 4), "skip_violations": r["violations"],
+            "note": "both 128 MiB buffers fit the 256 MB Infinity Cache: cache-assisted; hbm2048 is the "
+                    "HBM-bound figure"}
+    if traffic is not None:
+        if "error" in traffic:
+            roof["traffic_error"] = traffic["error"]
+        else:
+            roof["traffic"] = round(traffic["bytes"])
+            roof["traffic_detail"] = {
+                "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this window (--only boxes512-window, "
+                          "same --steps/--warmup), mean over its timed tick dispatches, scaled by a 1 GiB copy",
+                "read": round(traffic["read"]), "written": round(traffic["written"]),
+                "vs_algorithmic": round(traffic["bytes"] / alg_per_tick, 3), "scale": traffic["scale"]}
+    roof["dense"] = {"kernel": f"tick_kernel<{r['seg']},false> (every voxel, no skipping)",
+                     "kernel_ms": round(r["kern_ms"], 5), "achieved": round(ach_d / 1e9, 1), "frac": round(frac_d, 4)}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -410,27 +657,37 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded box scene generated in-process)",
         "config": {"workload": "boxes512: 512^3 uint8 world, seeded boxes (20% sand, 10% fluid, 5% static) "
-                               "over a floor; ticks in one HIP graph with a fused state hash every 50 ticks",
-                   "dims": [512, 512, 512], "hash_every": 50, "segment_bytes": r["seg"]},
-        "roofline": {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(frac, 4), "traffic": PMC_TRAFFIC_BOXES512,
-                     "traffic_source": "profiles/tick_kernel_boxes512.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                                       "bytes per tick)",
-                     "kernel": f"tick_kernel<{r['seg']},*,skip> ({sched} schedule), every tick of the timed region",
-                     "ms_per_tick_device": round(r["dev_ms"], 5),
-                     "alg_bytes_per_tick": int(alg_bytes / args.steps),
-                     "swept_fraction": round(r["swept"] / (vox * args.steps), 4),
-                     "skip_violations": r["violations"],
-                     "dense": {"kernel": f"tick_kernel<{r['seg']},false> (every voxel, no skipping)",
-                               "kernel_ms": round(r["kern_ms"], 5), "achieved": round(ach_d / 1e9, 1),
-                               "frac": round(frac_d, 4)}},
-        "final_hash": r["hashes"][-1] if r["hashes"] else None,
+                               "over a floor; ticks in HIP graphs with a fused state hash every 50 ticks and on "
+                               "the last timed tick",
+                   "dims": [512, 512, 512], "hash_every": HASH_EVERY, "segment_bytes": r["seg"]},
+        "roofline": roof,
+        "final_hash": r["hashes"][-1],
+        "final_tick": r["hash_ticks"][-1],
+        "parity": boxes512_parity(r["hashes"], r["hash_ticks"]),
     }
     if args.only is None:
         extra = {}
         p = bench_pillars2048(F, 200)
         ach2, frac2 = roofline(p["vox"], p["dense"]["kern_ms"])
         c = bench_copy(F, 8 << 30)
+        hbm = {"bound": "hbm", "achieved": round(ach2 / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+               "frac": round(frac2, 4), "traffic": None,
+               "kernel": f"tick_kernel<{p['dense']['seg']},false,false> (dense tick, 2048^3 pillar scene)",
+               "kernel_ms": round(p["dense"]["kern_ms"], 5), "alg_bytes_per_tick": 2 * p["vox"],
+               "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
+        if traffic2048 is not None:
+            if "error" in traffic2048:
+                hbm["traffic_error"] = traffic2048["error"]
+            else:
+                hbm["traffic"] = round(traffic2048["bytes"])
+                hbm["traffic_detail"] = {
+                    "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of {DENSE2048_WINDOW} dense ticks "
+                              "(--only dense2048-window), scaled by a 1 GiB copy",
+                    "read": round(traffic2048["read"]), "written": round(traffic2048["written"]),
+                    "vs_algorithmic": round(traffic2048["bytes"] / (2 * p["vox"]), 4)}
+        roof["hbm2048"] = hbm
+        fx_frames = golden_frames()
+        want200 = fx_frames[-1]["hash"] if fx_frames and fx_frames[-1]["tick"] == 200 else None
         pr = bench_pillars_render(F)
         extra["pillars2048_render"] = {
             "workload": "2048^3 pillars, 200 ticks in graphs of 10, 1920x1080 voxel render after each (20 frames), "
@@ -440,7 +697,7 @@ def main():
             "rays_per_s": round(pr["rays"] / (pr["render_ms"] * 1e-3), 1), "hit_fraction": round(pr["hit_frac"], 4),
             "voxel_steps_per_frame": round(pr["voxel_steps"]),
             "voxel_steps_per_s": round(pr["voxel_steps"] / (pr["render_ms"] * 1e-3), 1),
-            "final_hash": pr["hash"]}
+            "final_hash": pr["hash"], "parity": pr["parity"]}
         ps, pd = p["skip"], p["dense"]
         extra["pillars2048"] = {
             "workload": "2048^3 uint8, 256 seeded sand pillars in a fluid pool, 200 plain ticks in one graph",
@@ -454,7 +711,9 @@ def main():
                             "ms_per_tick": round(p["fused"]["kern_ms"], 4),
                             "achieved_GBps": round(p["vox"] * 2 / (2 * p["fused"]["kern_ms"] * 1e-3) / 1e9, 1),
                             "note": "achieved counts the 2 bytes per voxel that one pass moves for two ticks"},
-            "final_hash": pd["hash"], "segment_bytes": pd["seg"]}
+            "final_hash": pd["hash"], "segment_bytes": pd["seg"],
+            "parity": {"tick200_matches_oracle": (pd["hash"] == want200) if want200 is not None else None,
+                       "fixture": "tests/golden/pillars2048_frames.json"}}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4)}
         bb = bench_batched(F)
@@ -469,7 +728,7 @@ def main():
                                     "speedup_vs_plain": round(bb["all_ticks_speedup"], 2)},
             "settled_worlds": "a world with two quiet ticks in a row is a fixed point; its remaining ticks "
                               "are not run (only its final, hashed tick)",
-            "autotuned": bb["tuned"], "final_hash_digest": bb["hash_digest"]}
+            "autotuned": bb["tuned"], "final_hash_digest": bb["hash_digest"], "parity": bb["parity"]}
         sv = bench_serving(F)
         st = sv["stats"]
         extra["serving"] = {
@@ -479,7 +738,19 @@ def main():
             "tick_p50_ms": round(st["tick_p50_ms"], 4), "tick_p99_ms": round(st["tick_p99_ms"], 4),
             "request_p50_ms": round(st["req_p50_ms"], 2), "request_p99_ms": round(st["req_p99_ms"], 2),
             "device_busy_frac": round(sv["busy_frac"], 3), "peak_active": st["peak_active"],
-            "peak_bytes": st["peak_bytes"], "rejected": st["rejected"], "bit_exact_sample": sv["exact"]}
+            "peak_bytes": st["peak_bytes"], "rejected": st["rejected"],
+            "saturated": {"what": "the same requests submitted at once: the server's capacity",
+                          "wall_s": round(sv["saturated"]["wall"], 3),
+                          "gvox_per_s": round(sv["saturated"]["gvox_per_s"], 2),
+                          "tick_p50_ms": round(sv["saturated"]["stats"]["tick_p50_ms"], 4),
+                          "tick_p99_ms": round(sv["saturated"]["stats"]["tick_p99_ms"], 4),
+                          "peak_active": sv["saturated"]["stats"]["peak_active"]},
+            "plain": {"what": "one world per request (own grid, fill, one tick launch per tick, hash), "
+                              f"every 10th request ({sv['plain']['requests']})",
+                      "wall_s": round(sv["plain"]["wall"], 3), "gvox_per_s": round(sv["plain"]["gvox_per_s"], 2)},
+            "speedup_vs_plain": round(sv["speedup_vs_plain"], 2),
+            "saturated_speedup_vs_plain": round(sv["saturated_speedup_vs_plain"], 2),
+            "served_equals_alone": sv["exact"], "parity": sv["parity"]}
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -74,3 +74,36 @@ def test_voxel_steps_counts_the_walked_cells(O):
             est = bench.voxel_steps(cam, dims, hit, W, H)
             exact = _walk_cells(g, cam, W, H)
             assert abs(est - exact) <= max(2, exact // 200), (dims, ang, est, exact)
+
+
+def test_timed_window_always_hashes_its_last_tick():
+    import bench
+    for steps in (1, 20, 49, 50, 51, 123, 1000, 1001):
+        segs = bench.hash_segments(steps)
+        assert sum(nt for nt, _ in segs) == steps
+        assert all(nt % he == 0 for nt, he in segs)  # each run ends on a hashed tick
+        assert len(segs) <= 2 and (len(segs) == 1 or segs[0][0] % 2 == 0)
+
+
+def test_window_parity_flags_a_planted_mismatch():
+    import bench
+    fx = bench.golden_rows("boxes512_hashes.txt")
+    assert len(fx) == 2101
+    ticks = [55, 105, 125]
+    good = [fx[t][0] for t in ticks]
+    assert bench.boxes512_parity(good, ticks)["match"] is True
+    bad = list(good)
+    bad[-1] ^= 1
+    p = bench.boxes512_parity(bad, ticks)
+    assert p["match"] is False and p["mismatched_ticks"] == [125]
+    assert bench.boxes512_parity(good, [55, 105, 2101])["match"] is None  # beyond the fixture
+
+
+def test_boxes512_fixture_starts_at_the_oracle_fill(O):
+    import bench
+    fx = bench.golden_rows("boxes512_hashes.txt")
+    g, _ = O.scene_boxes((512, 512, 512), 1)
+    assert O.state_hash(g) == fx[0][0]
+    O.set_threads(0)
+    g, h = O.run(g, 0, 3, 1, use_lut=True)
+    assert [int(x) for x in h] == [fx[k][0] for k in (1, 2, 3)]
