@@ -171,7 +171,7 @@ def window_main(F, mode, steps, warmup):
 
 def pmc_traffic(mode, steps, warmup, kernel_sub, last_n):
     """HBM-side bytes per launch of the last `last_n` dispatches whose kernel
-    name contains `kernel_sub`, from two child runs of `--only mode` under
+    name contains one of `kernel_sub`, from two child runs of `--only mode` under
     rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  Each
     counter is scaled by what it reads for the copy of COPY_CAL_BYTES in the
     same child.  Runs before this process touches the GPU.  Returns a dict, or
@@ -219,7 +219,7 @@ def pmc_traffic(mode, steps, warmup, kernel_sub, last_n):
                     name, v = per.get(did, (r["Kernel_Name"], 0.0))
                     per[did] = (name, v + float(r["Counter_Value"]))
             disp = [per[k] for k in sorted(per)]
-            ticks = [v for name, v in disp if kernel_sub in name][-last_n:]
+            ticks = [v for name, v in disp if any(k in name for k in kernel_sub)][-last_n:]
             copies = [v for name, v in disp if "copy_kernel" in name]
             if len(ticks) != last_n or not copies:
                 return {"error": f"{ctr}: {len(ticks)} of {last_n} tick dispatches, {len(copies)} copies found"}
@@ -555,6 +555,9 @@ def cpu_baseline(seconds=12.0, omp_seconds=6.0):
 
 
 DENSE2048_WINDOW = 10  # dense 2048^3 ticks of the counter run for roofline.hbm2048
+# the dense single tick at 2048^3: 16-byte x-chunks (rows longer than 1024 bytes)
+DENSE2048_KERNELS = ("tick_chunk_kernel<", "tick_kernel<")
+DENSE2048_LABEL = "tick_chunk_kernel<16,false> (dense tick in 16-byte x-chunks, 2048^3 pillar scene)"
 
 
 def main():
@@ -577,9 +580,9 @@ def main():
     # the counter runs are child processes started before this one touches the GPU
     traffic = traffic2048 = None
     if args.only in (None, "boxes512") and not args.no_traffic:
-        traffic = pmc_traffic("boxes512-window", args.steps, args.warmup, "tick_kernel<", args.steps)
+        traffic = pmc_traffic("boxes512-window", args.steps, args.warmup, ("tick_kernel<",), args.steps)
         if args.only is None:
-            traffic2048 = pmc_traffic("dense2048-window", DENSE2048_WINDOW, 0, "tick_kernel<", DENSE2048_WINDOW)
+            traffic2048 = pmc_traffic("dense2048-window", DENSE2048_WINDOW, 0, DENSE2048_KERNELS, DENSE2048_WINDOW)
 
     import fallingsand3d_amd as F
     if F.device_count() < 1:
@@ -672,7 +675,7 @@ def main():
         c = bench_copy(F, 8 << 30)
         hbm = {"bound": "hbm", "achieved": round(ach2 / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                "frac": round(frac2, 4), "traffic": None,
-               "kernel": f"tick_kernel<{p['dense']['seg']},false,false> (dense tick, 2048^3 pillar scene)",
+               "kernel": DENSE2048_LABEL,
                "kernel_ms": round(p["dense"]["kern_ms"], 5), "alg_bytes_per_tick": 2 * p["vox"],
                "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
         if traffic2048 is not None:
@@ -711,7 +714,7 @@ def main():
                             "ms_per_tick": round(p["fused"]["kern_ms"], 4),
                             "achieved_GBps": round(p["vox"] * 2 / (2 * p["fused"]["kern_ms"] * 1e-3) / 1e9, 1),
                             "note": "achieved counts the 2 bytes per voxel that one pass moves for two ticks"},
-            "final_hash": pd["hash"], "segment_bytes": pd["seg"],
+            "final_hash": pd["hash"], "segment_bytes": pd["seg"], "dense_kernel": DENSE2048_LABEL,
             "parity": {"tick200_matches_oracle": (pd["hash"] == want200) if want200 is not None else None,
                        "fixture": "tests/golden/pillars2048_frames.json"}}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),

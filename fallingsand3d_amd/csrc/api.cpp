@@ -215,6 +215,14 @@ void plan_fused(fs3d_world *w) {
 // z row-quads per workgroup of the fused two-tick pass
 constexpr int kTick2ZLen = 32;
 
+// Dense single ticks of rows longer than 1024 bytes (with the automatic
+// segment) run one wave per 16-byte x-chunk of 1 KiB instead of one wave per
+// whole row quad (launch_tick_chunked): each wave moves a quarter of the bytes
+// at nx = 2048, and the tick takes 3.21 instead of 3.30 ms at 2048^3 (four
+// alternating runs each on one MI355X).  Shorter rows keep the row-quad kernel,
+// which packs several row quads into a wave.  0: the row-quad kernel.
+int dense_chunk_seg(const fs3d_world *w) { return w->seg_req == 0 && w->nx > 1024 ? 16 : 0; }
+
 // Enqueues nticks ticks starting from buffer w->cur; *final receives the index of
 // the buffer holding the result (a fused pair of dense ticks moves the state to
 // the other buffer once, a single tick once).  With launch = false it only
@@ -247,9 +255,14 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
                                     tick_dev, (u32)k, rule, w->stream, masks, w->d_stats, w->d_list, w->d_ctl,
                                     w->mask_words, w->pack_waves, kPackChunk));
       } else {
-        FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
-                             w->tick + (u64)k, tick_dev, (u32)k, rule,
-                             hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
+        if (!masks && dense_chunk_seg(w))
+          FS3D_HIP(launch_tick_chunked(dense_chunk_seg(w), w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                                       w->tick + (u64)k, tick_dev, (u32)k, rule, hashed ? w->d_parts : nullptr,
+                                       w->stream));
+        else
+          FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+                               w->tick + (u64)k, tick_dev, (u32)k, rule,
+                               hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
         if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
       }
       // once the world has been rendered: the bricks this tick changed

@@ -89,6 +89,10 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
                        u64 t, const u64 *tick_dev, u32 k_off, const u64 *rule,
                        u64 *hash_parts, hipStream_t s, u64 *const *masks = nullptr, u64 *stats = nullptr,
                        u32 *ctl = nullptr);
+// The dense tick (no skipping) with one wave per x-chunk of 64 * seg bytes of a
+// row quad (seg 8, 16 or 32; rows of any length up to kMaxNx).
+hipError_t launch_tick_chunked(int seg, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t, const u64 *tick_dev,
+                               u32 k_off, const u64 *rule, u64 *hash_parts, hipStream_t s);
 hipError_t launch_fill_u64(u64 *p, size_t n, u64 v, hipStream_t s);
 // Work-list entry of the listed skipping tick: active lanes of one row quad and
 // its position (zq << 32 | yq).

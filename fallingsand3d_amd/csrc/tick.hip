@@ -399,6 +399,67 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
   }
 }
 
+// ---- chunked dense tick ----------------------------------------------------------
+// The dense tick with one wave per x-chunk of 64 * S bytes of a row quad's four
+// rows, instead of one wave per whole rows: each wave moves fewer bytes (2 KiB
+// in and out at S = 8) and more waves share the rows.  A chunk's end lanes
+// read the neighbouring chunk's edge bytes from memory on odd ticks and
+// evaluate the straddling block from the same inputs as the neighbouring
+// chunk's end lane, each storing its own bytes.  Wave w of the grid takes
+// chunk w % chunks of row quad w / chunks along y.
+template <int S, bool HASH>
+__global__ __launch_bounds__(256) void tick_chunk_kernel(TickArgs a, int chunks) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const u64 t64 = tick_of(a);
+  const u32 t = (u32)t64;
+  const int p = (int)(t & 1u);
+  const int nyq = (a.ny + p + 1) >> 1, nzq = (a.nz + p + 1) >> 1;
+  const int LR = a.lanes_per_row;  // segments per row (any number)
+  const int w = (int)blockIdx.x * 4 + wv;
+  const int chunk = w % chunks, yq = w / chunks;
+  const int zq = (int)blockIdx.z * 65535 + (int)blockIdx.y;
+  const long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (!HASH && (yq >= nyq || zq >= nzq)) return;  // uniform per wave
+  const int seg = chunk * 64 + lane;
+  const bool active = seg < LR && yq < nyq && zq < nzq;
+  RowQuad<S> R;
+  R.seg = active ? seg : 0;
+  R.active = active;
+  R.yq = active ? yq : 0;
+  R.zq = active ? zq : 0;
+  R.q = 0;
+  R.skip = false;
+  R.nl = R.nr = R.sl = R.sr = false;
+  R.ml = active && p && lane == 0 && seg > 0;
+  R.mr = active && p && lane == 63 && seg < LR - 1;
+  rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, true);
+#pragma unroll
+  for (int r = 0; r < 4; r++) R.edge[r] = kStatic4;
+  if (R.ml || R.mr) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (R.valid >> r & 1) {
+        const u8 *row = a.src + R.rowoff[r];
+        const u32 l = R.ml ? row[-1] : 0u;
+        const u32 rt = R.mr ? row[S] : 0u;
+        R.edge[r] = (l << 24) | rt;
+      }
+    }
+  }
+  u64 acc = rq_update<S, HASH, false, false, true>(R, a.dst, LR, t, a.rule);
+  if constexpr (HASH) {
+    __shared__ u64 wsum[4];
+    acc = wave_sum_u64(acc);
+    if (lane == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const u64 tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      if (tot) atomicAdd((unsigned long long *)(a.parts + (size_t)(bid % kHashParts) * kHashStride),
+                         (unsigned long long)tot);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void hash_kernel(const u8 *g, size_t n, u64 *parts) {
   u64 acc = 0;
   const size_t nchunk = (n + 15) / 16;
@@ -538,6 +599,51 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
     case 32: return launch_s<32>(sh, a, hash, skip, s);
     default: return launch_s<64>(sh, a, hash, skip, s);
   }
+}
+
+hipError_t launch_tick_chunked(int seg, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t, const u64 *tick_dev,
+                               u32 k_off, const u64 *rule, u64 *hash_parts, hipStream_t s) {
+  if (seg != 8 && seg != 16 && seg != 32) return hipErrorInvalidValue;
+  TickArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.lanes_per_row = (nx + seg - 1) / seg;
+  a.rq_per_wave = 1;
+  a.lr_magic = 0;
+  a.t = t;
+  a.k_off = k_off;
+  a.tick_dev = tick_dev;
+  a.rule = rule;
+  a.parts = hash_parts;
+  a.masks[0] = a.masks[1] = nullptr;
+  a.stats = nullptr;
+  a.ctl = nullptr;
+  a.dilate = 0;
+  const int chunks = (a.lanes_per_row + 63) / 64;
+  // sized for the larger parity's row-quad counts (the kernel bounds itself)
+  const long long nyq_max = (ny + 2) >> 1, nzq_max = (nz + 2) >> 1;
+  const long long waves = nyq_max * chunks;
+  const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)std::min<long long>(nzq_max, 65535),
+                  (unsigned)((nzq_max + 65534) / 65535));
+  const bool hash = hash_parts != nullptr;
+  switch (seg) {
+    case 8:
+      if (hash) hipLaunchKernelGGL((tick_chunk_kernel<8, true>), grid, dim3(256), 0, s, a, chunks);
+      else hipLaunchKernelGGL((tick_chunk_kernel<8, false>), grid, dim3(256), 0, s, a, chunks);
+      break;
+    case 16:
+      if (hash) hipLaunchKernelGGL((tick_chunk_kernel<16, true>), grid, dim3(256), 0, s, a, chunks);
+      else hipLaunchKernelGGL((tick_chunk_kernel<16, false>), grid, dim3(256), 0, s, a, chunks);
+      break;
+    default:
+      if (hash) hipLaunchKernelGGL((tick_chunk_kernel<32, true>), grid, dim3(256), 0, s, a, chunks);
+      else hipLaunchKernelGGL((tick_chunk_kernel<32, false>), grid, dim3(256), 0, s, a, chunks);
+      break;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
