@@ -195,21 +195,17 @@ bool fuse_ok(const fs3d_world *w) { return !w->skip && w->fuse && tick2_supporte
 // Grids up to this many voxels keep both buffers within about twice the 256 MB
 // MALL, where the single-tick kernel is not HBM-bound and fusion's redundant
 // halo work costs more than the traffic it saves (512^3: 74.5 vs 46.5 us per
-// tick; 1024^3: 369 vs 402 us; 2048^3: 2.57 vs 3.26 ms).
+// tick; 1024^3: 369 vs 402 us; 2048^3: 2.40 vs 3.22 ms).
 constexpr u64 kFuseMinVoxels = 1ull << 28;
 
-// The fused pass's launch shape: the smallest segment that puts a whole row in
-// one wave (it shares no state with the single-tick kernel's shape), and the
-// default for pairing ticks.
+// The fused pass's launch shape for rows up to 1024 bytes: the smallest segment
+// that puts a whole row in one wave (it shares no state with the single-tick
+// kernel's shape).  Longer rows run in 16-byte x-chunks and need no shape.  The
+// default for pairing ticks is on for large grids whose rows the pass supports.
 void plan_fused(fs3d_world *w) {
-  for (int seg : {8, 16, 32, 64})
-    if ((w->nx + seg - 1) / seg <= 64) {
-      (void)plan_tick(w->nx, w->ny, w->nz, seg, &w->shape2);
-      break;
-    }
-  // 64-byte segments (nx > 2048) hold 211 VGPRs per lane: two waves per SIMD,
-  // too few to hide the rule-table gathers, so they are opt-in
-  w->fuse = (u64)w->nx * (u64)w->ny * (u64)w->nz > kFuseMinVoxels && w->shape2.seg <= 32;
+  const int seg = w->nx <= 512 ? 8 : 16;
+  (void)plan_tick(w->nx, w->ny, w->nz, seg, &w->shape2);
+  w->fuse = (u64)w->nx * (u64)w->ny * (u64)w->nz > kFuseMinVoxels && tick2_supported(w->shape2, w->nx, nullptr);
 }
 
 // z row-quads per workgroup of the fused two-tick pass

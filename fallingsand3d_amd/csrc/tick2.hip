@@ -152,15 +152,152 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
   }
 }
 
+// The fused pass for rows longer than one wave holds (nx > 1024): each tick-A
+// and tick-B row quad is split into x-chunks of 64 * S bytes, one wave each, so
+// a workgroup has (TY + 1) * chunks waves over the same LDS ring, and a wave's
+// registers hold S bytes of a row instead of a whole row's share.  Tick A's
+// chunk ends read the neighbouring chunk's edge bytes from HBM (odd ticks), tick
+// B's from the LDS rows the neighbouring chunk's tick-A wave wrote before the
+// barrier; in both, the straddling block is evaluated by both end lanes from
+// the same inputs and each stores its own bytes.  Wave w takes row quad w /
+// chunks and chunk w % chunks.
+template <int S>
+__global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
+  extern __shared__ __attribute__((aligned(16))) u8 lds[];
+  constexpr int ND = S / 4;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wv = w / chunks, c = w - wv * chunks;
+  const u64 t64 = a.tick_dev ? *a.tick_dev + a.k_off : a.t;
+  const u32 tA = (u32)t64, tB = tA + 1u;
+  const int pA = (int)(tA & 1u), pB = pA ^ 1;
+  const int nyqA = (a.ny + pA + 1) >> 1, nzqA = (a.nz + pA + 1) >> 1;
+  const int nyqB = (a.ny + pB + 1) >> 1, nzqB = (a.nz + pB + 1) >> 1;
+  const u32 G = gridDim.x, bid = blockIdx.x;
+  const u32 xcd = bid % kXcds, per = G / kXcds, extra = G % kXcds;
+  const int tile = (int)(xcd * per + min(xcd, extra) + bid / kXcds);
+  const int ytile = tile % a.ytiles, zseg = tile / a.ytiles;
+  const int Y0 = ytile * kTY;
+  const int YA0 = Y0 - 1 + pA;
+  const int y_lo = 2 * YA0 - pA + 1;
+  const int Z0 = zseg * a.zlen;
+  const int Z1 = min(Z0 + a.zlen, nzqB);
+  const int LR = a.lanes_per_row;  // S-byte segments per row
+  const size_t layer_bytes = (size_t)kLayerRows * a.stride;
+  const int s_first = Z0 + pA - 1, s_last = Z1 - 1 + pA;
+  const int yqA = YA0 + wv;
+  const int unused = (wv == 0 ? 0x5 : 0) | (wv == kWaves - 1 ? 0xA : 0);
+
+  // tick A's row quad (yqA, s), chunk c, fetched one step ahead of its update;
+  // the chunk ends' edge bytes come from HBM on odd ticks
+  auto fetch_a = [&](RowQuad<S> &R, int seg, int s) {
+    R.seg = seg;
+    R.yq = yqA;
+    R.zq = s;
+    R.active = seg < LR && yqA >= 0 && yqA < nyqA && s >= 0 && s < nzqA && s <= s_last;
+    R.q = 0;
+    R.skip = false;
+    R.nl = R.nr = R.sl = R.sr = false;
+    R.ml = R.active && pA && lane == 0 && seg > 0;
+    R.mr = R.active && pA && lane == 63 && seg < LR - 1;
+    rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, pA, true);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      R.edge[r] = kStatic4;
+      if ((R.ml || R.mr) && (R.valid >> r & 1)) {
+        const u8 *row = a.src + R.rowoff[r];
+        R.edge[r] = ((R.ml ? (u32)row[-1] : 0u) << 24) | (R.mr ? (u32)row[S] : 0u);
+      }
+    }
+  };
+  RowQuad<S> R;
+  fetch_a(R, c * 64 + lane, s_first);
+  for (int s = s_first; s <= s_last; s++) {
+    const int i = s - s_first;
+    // An opaque copy of the lane's segment index: otherwise the per-lane
+    // predicates derived from it are hoisted out of the loop into SGPR pairs.
+    int seg = c * 64 + lane;
+    asm volatile("" : "+v"(seg));
+    u8 *const prev_top = lds + (size_t)((2 * i + 2) % 3) * layer_bytes;  // top layer of A's z-row s - 1
+    u8 *const cur_bot = lds + (size_t)((2 * i) % 3) * layer_bytes;       // bottom layer of A's z-row s
+    u8 *const cur_top = lds + (size_t)((2 * i + 1) % 3) * layer_bytes;   // top layer of A's z-row s
+    RowQuad<S> Rn;
+    fetch_a(Rn, seg, s + 1);
+    // ---- tick A: row quad (yqA, s), chunk c, into LDS ----
+    R.valid &= ~unused;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = max(2 * wv + (r & 1) - 1, 0);
+      u8 *layer = (r >> 1) ? cur_top : cur_bot;
+      R.rowoff[r] = (size_t)(layer - lds) + (size_t)row * a.stride + (size_t)seg * S;
+    }
+    rq_update<S, false, false, false, true>(R, lds, LR, tA, a.rule);
+    __syncthreads();
+    // ---- tick B: row quad (Y0 + wv, s - pA), chunk c, from LDS into HBM ----
+    const int b = s - pA;
+    if (wv < kTY && b >= Z0 && b < Z1) {  // uniform per wave; no barrier inside
+      RowQuad<S> Q;
+      Q.seg = seg;
+      Q.yq = Y0 + wv;
+      Q.zq = b;
+      Q.active = seg < LR && Q.yq < nyqB;
+      Q.q = 0;
+      Q.skip = false;
+      Q.loaded = true;
+      Q.nl = Q.nr = Q.sl = Q.sr = false;
+      Q.ml = Q.active && pB && lane == 0 && seg > 0;
+      Q.mr = Q.active && pB && lane == 63 && seg < LR - 1;
+      const int y0 = 2 * Q.yq - pB, z0 = 2 * b - pB;
+      const int xs = Q.seg * S;
+      Q.m = min(S, a.nx - xs);
+      Q.valid = 0;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int y = y0 + (r & 1), z = z0 + (r >> 1);
+        const bool ok = Q.active && y >= 0 && y < a.ny && z >= 0 && z < a.nz;
+        Q.valid |= ok ? (1 << r) : 0;
+        Q.rowoff[r] = ((size_t)(ok ? z : 0) * a.ny + (ok ? y : 0)) * a.nx + xs;
+#pragma unroll
+        for (int j = 0; j < ND; j++) Q.own[r][j] = kStatic4;
+        Q.edge[r] = kStatic4;
+        if (ok) {
+          // z0 + 0 is the top layer of A's z-row s - 1, z0 + 1 the bottom of A's z-row s;
+          // y - y_lo = 2 * wv + (r & 1)
+          const u8 *lrow = ((r >> 1) ? cur_bot : prev_top) + (size_t)(y - y_lo) * a.stride;
+          load_seg<S>(lrow + xs, Q.own[r]);
+          if (Q.ml || Q.mr)
+            Q.edge[r] = ((Q.ml ? (u32)lrow[xs - 1] : 0u) << 24) | (Q.mr ? (u32)lrow[xs + S] : 0u);
+        }
+        if (Q.m < S) {
+#pragma unroll
+          for (int j = 0; j < ND; j++) Q.own[r][j] = keep_bytes(Q.own[r][j], Q.m - 4 * j);
+        }
+      }
+      rq_update<S, false, false, false, true>(Q, a.dst, LR, tB, a.rule);
+    }
+    __syncthreads();
+    R = Rn;
+  }
+}
+
 }  // namespace
 
 size_t tick2_stride(const TickShape &sh) { return ((size_t)sh.lanes_per_row * sh.seg + 15) & ~(size_t)15; }
 
+// Rows longer than this run the chunked fused pass (16-byte segments).
+constexpr int kTick2ChunkMinNx = 1024;
+constexpr int kTick2ChunkSeg = 16;
+
+size_t tick2c_stride(int nx) { return ((size_t)nx + 15) & ~(size_t)15; }
+
 bool tick2_supported(const TickShape &sh, int nx, size_t *lds_bytes) {
-  (void)nx;
-  if (sh.rq_per_wave != 1) return false;
-  const size_t stride = tick2_stride(sh);
-  const size_t bytes = 3 * (size_t)kLayerRows * stride;
+  size_t bytes = 0;
+  if (nx > kTick2ChunkMinNx) {
+    bytes = 3 * (size_t)kLayerRows * tick2c_stride(nx);
+  } else {
+    if (sh.rq_per_wave != 1) return false;
+    bytes = 3 * (size_t)kLayerRows * tick2_stride(sh);
+  }
   if (lds_bytes) *lds_bytes = bytes;
   return bytes <= 64 * 1024;
 }
@@ -187,12 +324,19 @@ hipError_t launch_tick2(const TickShape &sh, const u8 *src, u8 *dst, int nx, int
   a.tick_dev = tick_dev;
   a.rule = rule;
   const long long blocks = (long long)a.ytiles * ((nzq_max + zlen - 1) / zlen);
+  if (nx > kTick2ChunkMinNx) {
+    constexpr int S = kTick2ChunkSeg;
+    a.lanes_per_row = (nx + S - 1) / S;
+    a.stride = (int)tick2c_stride(nx);
+    const int chunks = (a.lanes_per_row + 63) / 64;  // 2 at nx = 2048, at most 4 within the LDS limit
+    hipLaunchKernelGGL((tick2c_kernel<S>), dim3((unsigned)blocks), dim3(kWaves * chunks * 64), lds, s, a, chunks);
+    return hipGetLastError();
+  }
   const dim3 grid((unsigned)blocks), block(kWaves * 64);
-  switch (sh.seg) {
+  switch (sh.seg) {  // nx <= 1024 in whole rows: 8 bytes up to 512, else 16
     case 8: hipLaunchKernelGGL((tick2_kernel<8>), grid, block, lds, s, a); break;
     case 16: hipLaunchKernelGGL((tick2_kernel<16>), grid, block, lds, s, a); break;
-    case 32: hipLaunchKernelGGL((tick2_kernel<32>), grid, block, lds, s, a); break;
-    default: hipLaunchKernelGGL((tick2_kernel<64>), grid, block, lds, s, a); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -131,10 +131,11 @@ int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode);
 /* Two-tick fusion of the dense tick (used when skipping is off): a pair of
  * ticks, neither hashed, runs as one pass that keeps the first tick's output in
  * LDS and writes only the second's, so the grid crosses HBM once per two ticks.
- * On by default for grids of more than 2^28 voxels with nx <= 2048 (smaller
+ * On by default for grids of more than 2^28 voxels that it supports (smaller
  * grids stay in the last-level cache, where single ticks are faster).  Needs
- * 256 < nx <= 3584 (whole rows per wave, LDS tiles within 64 KiB); get_fuse
- * reports whether fusion is in effect. */
+ * 256 < nx <= 3632 (rows up to 1024 bytes whole in a wave, longer ones in
+ * 16-byte x-chunks of 1 KiB; LDS tiles within 64 KiB); get_fuse reports
+ * whether fusion is in effect. */
 int fs3d_world_set_fuse(fs3d_world *w, int32_t on);
 int fs3d_world_get_fuse(const fs3d_world *w, int32_t *on);
 /* Sweep counters of the skipping tick: enable != 0 starts (or keeps) counting,

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(257, 9, 10), (300, 7, 66), (512, 12, 8), (513, 5, 70), (700, 13, 3), (1000, 8, 9), (1024, 2, 2),
           (1025, 11, 6), (2048, 6, 7), (2000, 3, 65), (333, 1, 1), (400, 31, 2),
-          (1500, 100, 80), (2100, 5, 7), (3584, 3, 4)]
+          (1500, 100, 80), (2100, 5, 7), (3584, 3, 4), (3632, 4, 5)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -61,9 +61,12 @@ def test_fuse_defaults_and_limits(gpu):
     with gpu.World((100, 8, 8), flags=gpu.DENSE) as w:
         w.fuse = True
         assert not w.fuse  # 100 bytes need 13 lanes of 8 bytes: several row quads per wave
-    with gpu.World((3585, 4, 4), flags=gpu.DENSE) as w:
+    with gpu.World((3632, 4, 4), flags=gpu.DENSE) as w:
         w.fuse = True
-        assert not w.fuse  # 57 lanes of 64 bytes: the LDS ring would exceed 64 KiB
+        assert w.fuse  # 16-byte x-chunks: 3 layers x 6 rows x 3632 bytes fit 64 KiB
+    with gpu.World((3633, 4, 4), flags=gpu.DENSE) as w:
+        w.fuse = True
+        assert not w.fuse  # 3 layers x 6 rows x 3648 bytes: the LDS ring would exceed 64 KiB
     with gpu.World((1024, 1024, 260), flags=gpu.DENSE) as w:
         assert w.fuse  # on by default above 2^28 voxels
     with gpu.World((600, 8, 8)) as w:
