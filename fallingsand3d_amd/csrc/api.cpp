@@ -132,7 +132,7 @@ constexpr size_t kStatWords = (size_t)kHashParts * kHashStride + 1;
 // the masks describe real changes again.
 int mark_all(fs3d_world *w) {
   w->occ_valid = false;  // a host-side change: the renderer rebuilds its maps
-  for (int i = 0; i < 2; i++) FS3D_HIP(launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream));
+  for (int i = 0; i < 2; i++) FS3D_HIP(launch_fill_u64(w->d_masks[i], 2 * w->mask_words, ~0ull, w->stream));
   // work-list counts start from zero after any host-side change (tick, shape)
   FS3D_HIP(hipMemsetAsync(w->d_ctl, 0, 4 * sizeof(u32), w->stream));
   return FS3D_OK;
@@ -151,7 +151,7 @@ int mark_cell(fs3d_world *w, int x, int y, int z) {
   u64 bits = 1ull << s;  // s <= 63
   if (s > 0) bits |= 1ull << (s - 1);
   if (s + 1 < lr) bits |= 1ull << (s + 1);
-  FS3D_HIP(launch_mark_cells(w->d_masks[0], w->d_masks[1], idx[0], idx[1], bits, w->stream));
+  FS3D_HIP(launch_mark_cells(w->d_masks[0], w->d_masks[1], idx[0], idx[1], bits, w->mask_words, w->stream));
   return FS3D_OK;
 }
 
@@ -355,7 +355,7 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   w->nz = dims[2];
   w->n = n;
   w->alloc = (n + kGuardBytes + 255) & ~(size_t)255;
-  w->mask_words = (size_t)((w->ny + 2) >> 1) * (size_t)((w->nz + 2) >> 1);
+  w->mask_words = mask_words_of(w->ny, w->nz);
   // work-list counts are 32-bit: beyond 2^31 row quads a world ticks densely
   w->skip = (flags & FS3D_WORLD_DENSE) || w->mask_words >= (1ull << 31) ? FS3D_SKIP_OFF : FS3D_SKIP_AUTO;
   rc = plan_tick(w->nx, w->ny, w->nz, 0, &w->shape);
@@ -365,7 +365,7 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->buf[i], w->alloc);
-  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->d_masks[i], w->mask_words * sizeof(u64));
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMalloc(&w->d_masks[i], 2 * w->mask_words * sizeof(u64));
   if (e == hipSuccess) e = hipMalloc(&w->d_list, w->mask_words * sizeof(TickEntry));
   if (e == hipSuccess) e = hipMalloc(&w->d_ctl, 4 * sizeof(u32));
   if (e == hipSuccess) e = hipMalloc(&w->d_parts, (size_t)kHashParts * kHashStride * sizeof(u64));
@@ -381,7 +381,7 @@ int fs3d_world_create(const int32_t dims[3], uint32_t flags, fs3d_world **out) {
   e = hipMemsetAsync(w->d_parts, 0, (size_t)kHashParts * kHashStride * sizeof(u64), w->stream);
   for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipMemsetAsync(w->buf[i], 0, w->alloc, w->stream);
   for (int i = 0; i < 2 && e == hipSuccess; i++)
-    e = launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream);
+    e = launch_fill_u64(w->d_masks[i], 2 * w->mask_words, ~0ull, w->stream);
   if (e == hipSuccess) e = hipMemsetAsync(w->d_ctl, 0, 4 * sizeof(u32), w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   if (e != hipSuccess) {
@@ -765,7 +765,7 @@ int fs3d_world_fill_boxes(fs3d_world *w, uint64_t seed, const fs3d_box_params *p
   }
   w->occ_valid = false;
   hipError_t e = hipMemcpyAsync(w->buf[w->cur], host, w->n, hipMemcpyHostToDevice, w->stream);
-  for (int i = 0; i < 2 && e == hipSuccess; i++) e = launch_fill_u64(w->d_masks[i], w->mask_words, ~0ull, w->stream);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) e = launch_fill_u64(w->d_masks[i], 2 * w->mask_words, ~0ull, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   (void)hipHostFree(host);
   if (e != hipSuccess) return set_error(FS3D_ERR_HIP, "box upload failed: %s", hipGetErrorString(e));

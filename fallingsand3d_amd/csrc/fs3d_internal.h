@@ -95,11 +95,19 @@ hipError_t launch_tick_chunked(int seg, const u8 *src, u8 *dst, int nx, int ny, 
                                u32 k_off, const u64 *rule, u64 *hash_parts, hipStream_t s);
 hipError_t launch_fill_u64(u64 *p, size_t n, u64 v, hipStream_t s);
 // Work-list entry of the listed skipping tick: active lanes of one row quad and
-// its position (zq << 32 | yq).
+// its position (zq << 32 | yq).  On odd ticks of the packed schedule, edge bit s
+// is set unless the block straddling x = s * seg is known not to change: it did
+// not change two ticks before, and none of its cells changed one tick before.
 struct TickEntry {
   u64 act;
   u64 pos;
+  u64 edge;
 };
+// Change masks of one parity: mask_words change words (bit s: a block that lane
+// segment s evaluated changed at the last tick of that parity), then mask_words
+// edge words (bit s: a block holding cell x = s * seg - 1 or x = s * seg changed
+// then; every writer may set more bits than that, never fewer).
+inline size_t mask_words_of(int ny, int nz) { return (size_t)((ny + 2) >> 1) * (size_t)((nz + 2) >> 1); }
 // Skipping tick in two launches: a plan pass over all row quads that lists
 // those with active lanes (list capacity mask_words), then a persistent pass of
 // list_waves waves (a multiple of 4) over the list.  No hash.  ctl holds three
@@ -123,7 +131,9 @@ hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int n
                               u64 *stats, TickEntry *list, u32 *ctl, size_t mask_words, u32 pack_waves,
                               int chunk);
 int pack_blocks_per_cu(int seg);
-hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s);
+// Sets change bits `bits` of row quad i0 (parity 0) and i1 (parity 1), and the
+// edge bits next to them (edge words at offset edge_off).
+hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, size_t edge_off, hipStream_t s);
 // Two dense ticks (t and t + 1) in one pass: src -> dst holds the state after
 // both.  Supported for one row quad per wave and LDS tiles within 64 KiB.
 bool tick2_supported(const TickShape &sh, int nx, size_t *lds_bytes);
@@ -182,7 +192,7 @@ struct fs3d_world {
   bool occ_valid = false;  // d_occ, updated at the bricks of d_dirty, describes the current grid
   double last_render_ms = 0.0;
   // settled-segment skipping: mode (FS3D_SKIP_*) and per-parity change masks
-  // (one u64 per row quad)
+  // (one change word and one edge word per row quad: 2 * mask_words u64)
   int skip = FS3D_SKIP_AUTO;
   bool fuse = false;  // dense ticks: pairs of ticks in one pass where supported
   fs3d::u64 *d_masks[2] = {nullptr, nullptr};

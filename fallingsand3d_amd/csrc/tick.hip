@@ -28,6 +28,7 @@ struct TickArgs {
   const u64 *rule;      // 4 x 65536 entries
   u64 *parts;           // hash partials (HASH only)
   u64 *masks[2];        // per-parity change masks (SKIP only)
+  size_t edge_off;      // edge words of the masks: masks[p] + edge_off (SKIP only)
   u64 *stats;           // optional sweep counters (SKIP only): kHashParts slots + violations
   u32 *ctl;             // work-list counts, one per tick mod 3 (SKIP only)
   int dilate;           // plan pass: odd ticks also list the neighbours of changed segments
@@ -80,7 +81,10 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, !SKIP || !R.skip || sk.load_skipped);
   if constexpr (SKIP && !HASH) {
     if (__ballot(R.active && !R.skip) == 0) {  // the whole wave is settled
-      if (R.active && R.seg == 0) sk.mine[R.q] = 0;
+      if (R.active && R.seg == 0) {
+        sk.mine[R.q] = 0;
+        sk.mine[a.edge_off + R.q] = 0;
+      }
       return;
     }
   }
@@ -88,7 +92,12 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   u64 acc = rq_update<S, HASH, SKIP>(R, a.dst, LR, t, a.rule, &ch);
   if constexpr (SKIP) {
     const u64 b = __ballot(ch);
-    if (R.active && R.seg == 0) sk.mine[R.q] = (b >> (lane - R.seg)) & lane_bits;
+    if (R.active && R.seg == 0) {
+      // edge bits: a changed segment's blocks touch its own edges only
+      const u64 m = (b >> (lane - R.seg)) & lane_bits;
+      sk.mine[R.q] = m;
+      sk.mine[a.edge_off + R.q] = (m | (m << 1)) & lane_bits;
+    }
     if (a.stats) {
       const u64 vox = R.active ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
       const u64 swept = wave_sum_u64(R.skip ? 0 : vox);
@@ -140,7 +149,11 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
   const long long nrq = (long long)nyq * nzq;
   const u64 lane_bits = a.lanes_per_row == 64 ? ~0ull : ((1ull << a.lanes_per_row) - 1ull);
   const u64 *other = a.masks[p ^ 1];
-  u64 act[kPlanPerThread], bal[kPlanPerThread];
+  // odd ticks of the packed schedule: which segment edges may have a changing
+  // straddling block (edge words of this parity's row quad, two ticks before,
+  // and of the four other-parity row quads sharing its rows, one tick before)
+  const bool edges = p && !a.dilate;
+  u64 act[kPlanPerThread], bal[kPlanPerThread], edg[kPlanPerThread];
   int yq[kPlanPerThread], zq[kPlanPerThread];
   __shared__ u32 wcount[kPlanPerThread * 4];
   __shared__ u32 wbase;
@@ -148,6 +161,7 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
   for (int i = 0; i < kPlanPerThread; i++) {
     const long long q = (long long)blockIdx.x * (256 * kPlanPerThread) + i * 256 + threadIdx.x;
     act[i] = 0;
+    edg[i] = 0;
     yq[i] = zq[i] = 0;
     if (q < nrq) {
       yq[i] = (int)(q % nyq);
@@ -165,6 +179,18 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
       u64 o = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) o |= in[k] ? v[k] : 0ull;
+      if (edges) {
+        u64 ev[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int yo = yq[i] + (k & 1) - p, zo = zq[i] + (k >> 1) - p;
+          ev[k] = other[a.edge_off + (in[k] ? (long long)zo * nyq_o + yo : 0)];
+        }
+        u64 ed = a.masks[p][a.edge_off + q];
+#pragma unroll
+        for (int k = 0; k < 4; k++) ed |= in[k] ? ev[k] : 0ull;
+        edg[i] = ed;
+      }
       // The neighbours' bits matter on odd ticks only (straddling blocks), and
       // only for the listed schedule: the packed one evaluates a straddling
       // block next to an unlisted segment from its true bytes instead.
@@ -190,6 +216,7 @@ __global__ __launch_bounds__(256) void plan_kernel(TickArgs a, TickEntry *list, 
       const u32 idx = wbase + wcount[i * 4 + wv] + (u32)__builtin_popcountll(bal[i] & ((1ull << lane) - 1ull));
       list[idx].act = act[i];
       list[idx].pos = ((u64)(u32)zq[i] << 32) | (u32)yq[i];
+      if (!a.dilate) list[idx].edge = edg[i];
     }
   }
 }
@@ -245,7 +272,11 @@ __global__ __launch_bounds__(256) void tick_list_kernel(TickArgs a, const TickEn
     bool ch = false;
     rq_update<S, false, true>(R, a.dst, LR, t, a.rule, &ch);
     const u64 b = __ballot(ch);
-    if (active && seg == 0) mine[R.q] = (b >> (lane - seg)) & lane_bits;
+    if (active && seg == 0) {
+      const u64 m = (b >> (lane - seg)) & lane_bits;
+      mine[R.q] = m;
+      mine[a.edge_off + R.q] = (m | (m << 1)) & lane_bits;
+    }
     if (a.stats) {
       const u64 vox = active && !R.skip ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
       const u64 swept = wave_sum_u64(vox);
@@ -260,26 +291,50 @@ __global__ __launch_bounds__(256) void tick_list_kernel(TickArgs a, const TickEn
 // The listed tick gives each listed row quad a whole wave, which idles most lanes
 // when activity is sparse within rows, and a wave pays for the block update
 // whatever its active lanes.  This kernel reads the same list (same plan pass),
-// a chunk of up to 64 entries per wave, and packs their active lane segments
-// into full waves of 64 items: item
-// i of the chunk is the k-th active segment of entry o, with o found by a binary
-// search over the lanes' inclusive prefix of popcounts and the segment by a
-// select of the k-th set bit.  Items are in (entry, segment) order, so a lane's
-// left neighbour segment is active exactly when the entry's bit seg-1 is set,
-// and then it sits in the previous lane (or, for lane 0, in the previous group).
+// a chunk of entries per wave, and packs their active 64-byte sectors into full
+// waves: a sector of the four rows is one item of G = 64 / S lanes (S < 64), each
+// lane one S-byte segment of it, so every item loads and stores whole 64-byte
+// sectors.  HBM moves lines of 128 bytes for reads and sectors of 64 bytes for
+// writes: a store of part of a sector costs more than a store of all of it (at
+// 8 % of the 32-byte segments of a 2048^3 grid, scattered over its row quads,
+// writing their whole sectors took 1.14 ms against 1.50 ms for the segments
+// alone), and a sector's other segment, settled or not, comes with the same
+// line.  Item i of the chunk is the k-th active sector of entry o, with o found
+// by a binary search over the lanes' inclusive prefix of popcounts and the
+// sector by a select of the k-th set bit.  Items are in (entry, sector) order,
+// so a lane's left neighbour segment is being updated exactly when it is in the
+// same item or the entry's sector bit to the left is set, and then it sits in
+// the previous lane (or, for lane 0, in the previous group).
 // On odd ticks a straddling block always sees its true bytes: the neighbour's
 // edge bytes come from the adjacent lane (DPP) when it holds that segment, and
 // from memory otherwise.  The plan pass therefore lists no neighbours of changed
-// segments (unlike the listed schedule), and a lane next to an unlisted segment
-// stores that segment's cell of the straddling block itself when it changes,
-// setting the segment's change bit.  Change bits are ORed per entry within a
-// group (segmented scan) and routed to the entry's lane, which writes the
-// entry's mask word after the last group.
+// segments (unlike the listed schedule), and a lane next to a sector that is not
+// listed stores that sector's cell of the straddling block itself when it
+// changes, setting the segment's change bit.  A settled segment evaluated with
+// its sector keeps its bytes and its change bit stays clear.  Change bits are
+// ORed per entry within a group (segmented scan) and routed to the entry's
+// lane, which writes the entry's mask word after the last group.
 
-template <int S>
+// Bit j: one of the G segment bits G*j .. G*j + G - 1 of act is set.
+template <int G>
+__device__ __forceinline__ u64 sector_bits(u64 act) {
+  if constexpr (G == 1) {
+    return act;
+  } else {
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < 64 / G; j++) s |= (u64)(((act >> (G * j)) & ((1ull << G) - 1ull)) != 0ull) << j;
+    return s;
+  }
+}
+
+template <int S, int G>
 __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEntry *__restrict__ list,
                                                         const u32 *ctl, u32 nwaves, int chunk) {
+  static_assert(G >= 1 && 64 % G == 0, "lanes per item divide the wave");
+  constexpr int kItems = 64 / G;  // items per group of 64 lanes
   const int lane = threadIdx.x & 63;
+  const int h = lane & (G - 1);  // segment of the lane within its item
   const u32 wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const u64 t64 = tick_of(a);
   const u32 t = (u32)t64;
@@ -290,12 +345,14 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
   const u64 lane_bits = LR == 64 ? ~0ull : ((1ull << LR) - 1ull);
   u64 *mine = a.masks[p];
   for (u64 c0 = (u64)wave * chunk; c0 < count; c0 += (u64)nwaves * chunk) {
-    // this lane's entry of the chunk, and the prefix of active segments
+    // this lane's entry of the chunk, and the prefix of active sectors
     const u64 e = c0 + lane;
     const bool has = lane < chunk && e < count;
     const u64 act = has ? list[e].act : 0ull;
     const u64 pos = has ? list[e].pos : 0ull;
-    const int n = __builtin_popcountll(act);
+    const u64 edge = (has && p) ? list[e].edge : 0ull;
+    const u64 sec = sector_bits<G>(act);
+    const int n = __builtin_popcountll(sec);
     int incl = n;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -304,11 +361,11 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
     }
     const int excl = incl - n;
     const int total = __shfl(incl, 63, 64);
-    u64 acc = 0;  // change bits of this lane's entry
-    for (int g0 = 0; g0 < total; g0 += 64) {
+    u64 acc = 0, acc_e = 0;  // change and edge bits of this lane's entry
+    for (int g0 = 0; g0 < total; g0 += kItems) {
       // An opaque copy of the item index: otherwise predicates derived from the
       // lane index are hoisted out of the loop into SGPR pairs.
-      int idx = g0 + lane;
+      int idx = g0 + lane / G;
       asm volatile("" : "+v"(idx));
       const bool active = idx < total;
       // owner: the number of entries whose inclusive prefix is <= idx
@@ -320,38 +377,50 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       }
       o = active ? o : 63;
       int k = idx - __shfl(excl, o, 64);
-      const u64 ao = ((u64)(u32)__shfl((int)(act >> 32), o, 64) << 32) | (u32)__shfl((int)(u32)act, o, 64);
+      const u64 ao = ((u64)(u32)__shfl((int)(sec >> 32), o, 64) << 32) | (u32)__shfl((int)(u32)sec, o, 64);
       const u32 yq = (u32)__shfl((int)(u32)pos, o, 64), zq = (u32)__shfl((int)(pos >> 32), o, 64);
-      // the k-th set bit of ao
-      int seg = 0;
+      u64 eo = 0;
+      if (p) eo = ((u64)(u32)__shfl((int)(edge >> 32), o, 64) << 32) | (u32)__shfl((int)(u32)edge, o, 64);
+      // the k-th set bit of ao: the item's sector
+      int sj = 0;
 #pragma unroll
       for (int w = 32; w >= 1; w >>= 1) {
-        const int c = __builtin_popcountll((ao >> seg) & ((1ull << w) - 1ull));
+        const int c = __builtin_popcountll((ao >> sj) & ((1ull << w) - 1ull));
         if (k >= c) {
           k -= c;
-          seg += w;
+          sj += w;
         }
       }
-      seg = active ? seg : 0;
-      const bool left_act = active && seg > 0 && ((ao >> (seg - 1)) & 1ull);
-      const bool right_act = active && seg < LR - 1 && ((ao >> (seg + 1)) & 1ull);
+      const int seg0 = sj * G + h;
+      const bool lv = active && seg0 < LR;  // the lane holds a segment of the row
+      const int seg = lv ? seg0 : 0;
+      // neighbour segments being updated in this group or another one
+      const bool left_act = lv && seg > 0 && (h > 0 || ((ao >> (sj - 1)) & 1ull));
+      const bool right_act = lv && seg < LR - 1 && (h < G - 1 || ((ao >> (sj + 1)) & 1ull));
       RowQuad<S> R;
       R.seg = seg;
-      R.active = active;
+      R.active = lv;
       R.yq = (int)yq;
       R.zq = (int)zq;
       R.q = (long long)zq * nyq + yq;
       R.skip = false;
       // No keep-input: on odd ticks a straddling block always sees its true
       // bytes, from the adjacent lane when it holds the neighbour segment and
-      // from memory otherwise; next to a segment that is not listed this lane
+      // from memory otherwise; next to a sector that is not listed this lane
       // also stores the neighbour's cell of that block.
-      R.nl = false;
-      R.nr = false;
-      R.ml = active && seg > 0 && (lane == 0 || !left_act);
-      R.mr = active && seg < LR - 1 && (lane == 63 || !right_act);
-      R.sl = active && seg > 0 && !left_act;
-      R.sr = active && seg < LR - 1 && !right_act;
+      // A straddling block that is known not to change (its edge bit is clear
+      // on an odd tick) keeps its input instead: nothing is read from the
+      // neighbour, whose line then often stays unfetched.
+      const bool lfix = p && !((eo >> seg) & 1ull);
+      const bool rfix = p && seg < 63 && !((eo >> (seg + 1)) & 1ull);
+      const bool lmem = lv && seg > 0 && (lane == 0 || !left_act);
+      const bool rmem = lv && seg < LR - 1 && (lane == 63 || !right_act);
+      R.nl = lmem && lfix;
+      R.nr = rmem && rfix;
+      R.ml = lmem && !lfix;
+      R.mr = rmem && !rfix;
+      R.sl = lv && seg > 0 && !left_act && !lfix;
+      R.sr = lv && seg < LR - 1 && !right_act && !rfix;
       rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, true);
 #pragma unroll
       for (int r = 0; r < 4; r++) R.edge[r] = 0;
@@ -373,29 +442,45 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch, &echg);
       // OR of the change bits of each entry's run of lanes in this group (a
       // neighbour's cell changed by this lane sets the neighbour's bit) ...
-      u64 bits = (active && ch) ? (1ull << seg) : 0ull;
+      u64 bits = (lv && ch) ? (1ull << seg) : 0ull;
       if (echg & 1u) bits |= 1ull << (seg - 1);
       if (echg & 2u) bits |= 1ull << (seg + 1);
+      // edge bits: the lane's first block touches edge seg, its last edge seg + 1
+      u64 ebits = 0;
+      if (lv && (echg & 4u)) ebits |= 1ull << seg;
+      if (lv && (echg & 8u) && seg < 63) ebits |= 1ull << (seg + 1);
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const u64 ob = (u64)__shfl_up((unsigned long long)bits, d, 64);
+        const u64 oe = (u64)__shfl_up((unsigned long long)ebits, d, 64);
         const int oo = __shfl_up(o, d, 64);
-        if (lane >= d && oo == o) bits |= ob;
+        if (lane >= d && oo == o) {
+          bits |= ob;
+          ebits |= oe;
+        }
       }
       // ... read by the entry's lane from the run's last lane
-      const int s0 = max(excl, g0), s1 = min(incl, g0 + 64);
-      const int src = s1 > s0 ? s1 - 1 - g0 : lane;
+      const int s0 = max(excl, g0), s1 = min(incl, g0 + kItems);
+      const int src = s1 > s0 ? (s1 - g0) * G - 1 : lane;
       const u64 run = (u64)__shfl((unsigned long long)bits, src, 64);
-      if (s1 > s0) acc |= run;
+      const u64 run_e = (u64)__shfl((unsigned long long)ebits, src, 64);
+      if (s1 > s0) {
+        acc |= run;
+        acc_e |= run_e;
+      }
       if (a.stats) {
-        const u64 vox = active ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
+        const u64 vox = lv ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
         const u64 swept = wave_sum_u64(vox);
         if (lane == 0 && swept)
           atomicAdd((unsigned long long *)(a.stats + (size_t)(wave % kHashParts) * kHashStride),
                     (unsigned long long)swept);
       }
     }
-    if (has) mine[(long long)(u32)(pos >> 32) * nyq + (u32)pos] = acc & lane_bits;
+    if (has) {
+      const long long q = (long long)(u32)(pos >> 32) * nyq + (u32)pos;
+      mine[q] = acc & lane_bits;
+      mine[a.edge_off + q] = acc_e & lane_bits;
+    }
   }
 }
 
@@ -538,10 +623,12 @@ __global__ __launch_bounds__(256) void fill_u64_kernel(u64 *p, size_t n, u64 v) 
     p[i] = v;
 }
 
-__global__ void mark_cells_kernel(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits) {
+__global__ void mark_cells_kernel(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, size_t edge_off) {
   if (threadIdx.x == 0) {
     m0[i0] |= bits;
     m1[i1] |= bits;
+    m0[edge_off + i0] |= bits | (bits << 1);
+    m1[edge_off + i1] |= bits | (bits << 1);
   }
 }
 
@@ -588,6 +675,7 @@ hipError_t launch_tick(const TickShape &sh, const u8 *src, u8 *dst, int nx, int 
   a.parts = hash_parts;
   a.masks[0] = masks ? masks[0] : nullptr;
   a.masks[1] = masks ? masks[1] : nullptr;
+  a.edge_off = mask_words_of(ny, nz);
   a.stats = masks ? stats : nullptr;
   a.ctl = masks ? ctl : nullptr;
   a.dilate = 1;
@@ -619,6 +707,7 @@ hipError_t launch_tick_chunked(int seg, const u8 *src, u8 *dst, int nx, int ny, 
   a.rule = rule;
   a.parts = hash_parts;
   a.masks[0] = a.masks[1] = nullptr;
+  a.edge_off = 0;
   a.stats = nullptr;
   a.ctl = nullptr;
   a.dilate = 0;
@@ -666,6 +755,7 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.parts = nullptr;
   a.masks[0] = masks[0];
   a.masks[1] = masks[1];
+  a.edge_off = mask_words;
   a.stats = stats;
   a.ctl = ctl;
   a.dilate = 1;
@@ -680,6 +770,15 @@ hipError_t launch_tick_listed(const TickShape &sh, const u8 *src, u8 *dst, int n
   }
   return hipGetLastError();
 }
+
+// Lanes per item of the packed tick: one item per 64-byte sector of the rows
+// for 16- and 32-byte segments (2048^3 pillar scene, 32-byte segments: 0.976
+// against 1.092 ms per tick over ticks 0-60 with one lane per item).  8-byte
+// segments keep one lane per item: eight lanes per item took 0.081 against
+// 0.072 ms per tick on the 512^3 box scene (ticks 5-25), whose two buffers sit
+// mostly in the last-level cache.
+constexpr int kPackSectorBytes = 64;
+constexpr int pack_g(int seg) { return seg >= kPackSectorBytes || seg < 16 ? 1 : kPackSectorBytes / seg; }
 
 hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int nx, int ny, int nz, u64 t,
                               const u64 *tick_dev, u32 k_off, const u64 *rule, hipStream_t s, u64 *const *masks,
@@ -701,6 +800,7 @@ hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int n
   a.parts = nullptr;
   a.masks[0] = masks[0];
   a.masks[1] = masks[1];
+  a.edge_off = mask_words;
   a.stats = stats;
   a.ctl = ctl;
   a.dilate = 0;
@@ -708,10 +808,10 @@ hipError_t launch_tick_packed(const TickShape &sh, const u8 *src, u8 *dst, int n
   hipLaunchKernelGGL(plan_kernel, dim3((unsigned)plan_blocks), dim3(256), 0, s, a, list, ctl);
   const dim3 grid(pack_waves / 4);
   switch (sh.seg) {
-    case 8: hipLaunchKernelGGL((tick_pack_kernel<8>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
-    case 16: hipLaunchKernelGGL((tick_pack_kernel<16>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
-    case 32: hipLaunchKernelGGL((tick_pack_kernel<32>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
-    default: hipLaunchKernelGGL((tick_pack_kernel<64>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    case 8: hipLaunchKernelGGL((tick_pack_kernel<8, pack_g(8)>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    case 16: hipLaunchKernelGGL((tick_pack_kernel<16, pack_g(16)>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    case 32: hipLaunchKernelGGL((tick_pack_kernel<32, pack_g(32)>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
+    default: hipLaunchKernelGGL((tick_pack_kernel<64, pack_g(64)>), grid, dim3(256), 0, s, a, list, ctl, pack_waves, chunk); break;
   }
   return hipGetLastError();
 }
@@ -720,10 +820,10 @@ int pack_blocks_per_cu(int seg) {
   int n = 0;
   hipError_t e;
   switch (seg) {
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<8>, 256, 0); break;
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<16>, 256, 0); break;
-    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<32>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<64>, 256, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<8, pack_g(8)>, 256, 0); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<16, pack_g(16)>, 256, 0); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<32, pack_g(32)>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tick_pack_kernel<64, pack_g(64)>, 256, 0); break;
   }
   return (e == hipSuccess && n > 0) ? n : 1;
 }
@@ -748,8 +848,8 @@ hipError_t launch_fill_u64(u64 *p, size_t n, u64 v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, hipStream_t s) {
-  hipLaunchKernelGGL(mark_cells_kernel, dim3(1), dim3(64), 0, s, m0, m1, i0, i1, bits);
+hipError_t launch_mark_cells(u64 *m0, u64 *m1, long long i0, long long i1, u64 bits, size_t edge_off, hipStream_t s) {
+  hipLaunchKernelGGL(mark_cells_kernel, dim3(1), dim3(64), 0, s, m0, m1, i0, i1, bits, edge_off);
   return hipGetLastError();
 }
 

@@ -264,7 +264,9 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // P = t & 1 when known at compile time (even ticks need no neighbour exchange,
 // no shifted frame and no extra frame dword), or -1: the parity is read from t.
 // With EDGES, *edge_changed receives bit 0 / bit 1 when this lane stored the
-// left / right neighbour's cell (R.sl / R.sr) because it changed.
+// left / right neighbour's cell (R.sl / R.sr) because it changed, and bit 2 /
+// bit 3 when the lane's first / last block changed (on odd ticks the blocks
+// straddling its left / right edge).
 template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P>
 __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
                                            bool *changed, u32 *edge_changed) {
@@ -323,6 +325,7 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
     nr = p && seg < LR - 1 && fr_ != 0;
   }
   u32 dif = 0;
+  u32 ends = 0;  // EDGES: bit 0 / 1: the first / last block changed
 
   // ---- variants: one 32-bit word per 16 consecutive blocks ----
   // A lane evaluates blocks bx0 .. bx0 + S/2 (the last one on odd ticks only);
@@ -384,6 +387,12 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
     bool keep_in = false;
     if constexpr (SKIP) keep_in = (j == 0 && nl) || (j == ND && nr);  // block 0 straddles into a settled lane
     if constexpr (SKIP || TRACK) dif |= (keep_in ? lo1 : (lo0 | lo1)) & (u32)kRuleChanged;
+    if constexpr (EDGES) {
+      // the first block is block 0 of dword 0; the last is block 0 of the extra
+      // dword on odd ticks and block 1 of dword ND - 1 on even ones
+      if (j == 0) ends |= ((keep_in ? 0u : lo0) & (u32)kRuleChanged) ? 1u : 0u;
+      if (j == (p ? ND : ND - 1)) ends |= ((p ? (keep_in ? 0u : lo0) : lo1) & (u32)kRuleChanged) ? 2u : 0u;
+    }
     out[0][j] = __builtin_amdgcn_perm(lo1, lo0, 0x05040100u) & 0x7F7F7F7Fu;
     out[1][j] = __builtin_amdgcn_perm(lo1, lo0, 0x07060302u);
     out[2][j] = __builtin_amdgcn_perm(hi1, hi0, 0x05040100u);
@@ -428,7 +437,7 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
         }
       }
     }
-    if (edge_changed) *edge_changed = echg;
+    if (edge_changed) *edge_changed = echg | (ends << 2);
   }
 
   // ---- back to own bytes, restore reserved values, store, hash ----
