@@ -377,10 +377,21 @@ def voxel_steps(cam, dims, hit, W, H):
     return int(steps[inside].sum())
 
 
-def bench_copy(F, nbytes, iters=20):
+def bench_copy(F, nbytes, iters=20, scatter=True):
+    """The streaming copy, and the packed skipping tick's access shape without
+    its arithmetic: 5 scattered 32-byte segments of every row quad of a 2048^3
+    grid pair (about the tick's density on the pillar scene), copied as 32-byte
+    pieces and as whole 64-byte sectors."""
     ms = F.copy_probe(nbytes, iters)
     log(f"[bench] copy: {nbytes} bytes in {ms:.4f} ms ({2 * nbytes / ms / 1e6:.1f} GB/s)")
-    return dict(nbytes=nbytes, ms=ms, GBps=2 * nbytes / (ms * 1e-3) / 1e9)
+    out = dict(nbytes=nbytes, ms=ms, GBps=2 * nbytes / (ms * 1e-3) / 1e9)
+    if scatter:
+        for sector in (False, True):
+            sms, sb = F.scatter_probe(5, sector)
+            key = "scatter5_sector64" if sector else "scatter5_seg32"
+            out[key] = dict(ms=round(sms, 4), item_bytes=sb, GBps=round(sb / (sms * 1e-3) / 1e9, 1))
+            log(f"[bench] {key}: {sb / 1e9:.3f} GB of items in {sms:.4f} ms")
+    return out
 
 
 def bench_batched(F, nworlds=4096, ticks=500, plain_ticks=20, seed=3):
@@ -718,7 +729,8 @@ def main():
             "parity": {"tick200_matches_oracle": (pd["hash"] == want200) if want200 is not None else None,
                        "fixture": "tests/golden/pillars2048_frames.json"}}
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
-                         "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4)}
+                         "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4),
+                         "scatter5_seg32": c.get("scatter5_seg32"), "scatter5_sector64": c.get("scatter5_sector64")}
         bb = bench_batched(F)
         extra["batched"] = {
             "workload": "4096 worlds, edges log-uniform 16-128, rain fill, density U(0.05,0.5), 500 ticks",

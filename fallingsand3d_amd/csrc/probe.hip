@@ -22,6 +22,48 @@ __global__ __launch_bounds__(256) void copy_kernel(const u32x4 *__restrict__ src
   if (i < n16) dst[i] = src[i];
 }
 
+__device__ __forceinline__ u32 probe_mix(u32 h) {
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// The access shape of the packed skipping tick without its arithmetic: item i
+// is the k-th of `per_rq` (k = i % per_rq) 32-byte segments of row quad i /
+// per_rq of an n^3 grid (even parity: rows (2yq, 2yq + 1) x (2zq, 2zq + 1)), at
+// segment k * span + h, span = floor(64 / per_rq), h a hash in [0, span), so the
+// segments are distinct, ascending and scattered.  One lane per item copies
+// the four rows' 32 bytes (SECTOR false) or their whole 64-byte sectors
+// (SECTOR true) from src to dst.  n = 2048: 64 segments per row.
+template <bool SECTOR>
+__global__ __launch_bounds__(256) void scatter_kernel(const u8 *__restrict__ src, u8 *__restrict__ dst, int n,
+                                                      size_t nitems, int per_rq) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nitems) return;
+  const size_t rq = i / (size_t)per_rq;
+  const int k = (int)(i % (size_t)per_rq);
+  const int span = 64 / per_rq;
+  const int seg = k * span + (int)(probe_mix((u32)rq * 131u + (u32)k) % (u32)span);
+  const int nrq_y = n / 2;
+  const size_t yq = rq % (size_t)nrq_y, zq = rq / (size_t)nrq_y;
+  const size_t base = ((2 * zq) * (size_t)n + 2 * yq) * (size_t)n;
+  const size_t off[4] = {base, base + n, base + (size_t)n * n, base + (size_t)n * n + n};
+  constexpr int kV = SECTOR ? 4 : 2;  // 16-byte vectors per row
+  const size_t x = SECTOR ? (size_t)(seg >> 1) * 64 : (size_t)seg * 32;
+  u32x4 v[4][kV];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < kV; c++) v[r][c] = *(const u32x4 *)(src + off[r] + x + 16 * c);
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < kV; c++) *(u32x4 *)(dst + off[r] + x + 16 * c) = v[r][c];
+}
+
 }  // namespace
 }  // namespace fs3d
 
@@ -67,5 +109,51 @@ extern "C" int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy
   if (b) (void)hipFree(b);
   if (first != hipSuccess) return set_error(FS3D_ERR_HIP, "copy probe failed: %s", hipGetErrorString(first));
   *ms_per_copy = (double)ms / iters;
+  return clear_error();
+}
+
+extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass,
+                                  double *item_bytes) {
+  constexpr int n = 2048;
+  if (!ms_per_pass || !item_bytes || iters < 1 || per_rq < 1 || per_rq > 64)
+    return set_error(FS3D_ERR_INVALID, "scatter probe needs 1 <= per_rq <= 64 and iters >= 1");
+  const size_t nbytes = (size_t)n * n * n;
+  const size_t nitems = (size_t)(n / 2) * (n / 2) * (size_t)per_rq;
+  const unsigned blocks = (unsigned)((nitems + 255) / 256);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (int rc = ensure_device(dev)) return rc;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, nbytes);
+  if (e == hipSuccess) e = hipMalloc(&b, nbytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 1, nbytes, s);
+  if (e == hipSuccess) e = hipMemsetAsync(b, 0, nbytes, s);
+  for (int i = 0; i <= iters && e == hipSuccess; i++) {  // launch 0 warms up
+    if (i == 1) e = hipEventRecord(e0, s);
+    if (e != hipSuccess) break;
+    if (sector)
+      hipLaunchKernelGGL(scatter_kernel<true>, dim3(blocks), dim3(256), 0, s, (const u8 *)a, (u8 *)b, n, nitems, per_rq);
+    else
+      hipLaunchKernelGGL(scatter_kernel<false>, dim3(blocks), dim3(256), 0, s, (const u8 *)a, (u8 *)b, n, nitems, per_rq);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  const hipError_t first = e;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (first != hipSuccess) return set_error(FS3D_ERR_HIP, "scatter probe failed: %s", hipGetErrorString(first));
+  *ms_per_pass = (double)ms / iters;
+  *item_bytes = (double)nitems * 4 * (sector ? 64 : 32) * 2;
   return clear_error();
 }

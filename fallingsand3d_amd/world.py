@@ -50,6 +50,15 @@ def copy_probe(nbytes, iters=20):
     return ms.value
 
 
+def scatter_probe(per_rq, sector, iters=10):
+    """(ms per pass, item bytes) of the packed tick's access shape on a 2048^3
+    grid pair: per_rq scattered 32-byte segments of every row quad (sector:
+    their 64-byte sectors) read from one grid and written to the other."""
+    ms, nb = c_double(0), c_double(0)
+    check(lib().fs3d_scatter_probe(per_rq, 1 if sector else 0, iters, ctypes.byref(ms), ctypes.byref(nb)))
+    return ms.value, nb.value
+
+
 def threshold(num, den):
     t = c_u64(0)
     check(lib().fs3d_threshold(num, den, ctypes.byref(t)))

@@ -421,6 +421,14 @@ int fs3d_world_last_run_ms(const fs3d_world *w, double *ms_per_tick);
  * compared with, and the known-size kernel that scales rocprof byte counters.
  * No reference counterpart (the reference has no timing at all). */
 int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy);
+/* Mean device time (ms) of one pass of the packed skipping tick's access
+ * shape without its arithmetic, on a 2048^3 grid pair (16 GiB, allocated and
+ * freed by the call): per_rq (1..64) scattered, distinct 32-byte segments
+ * of every row quad, four rows each, copied from one grid to the other by one
+ * lane per item; sector != 0 copies each segment's whole 64-byte sector
+ * instead.  *item_bytes receives the bytes the items read and write.  No
+ * reference counterpart. */
+int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass, double *item_bytes);
 
 #ifdef __cplusplus
 }

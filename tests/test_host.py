@@ -84,6 +84,16 @@ def test_invalid_dims_rejected_before_device():
         F.World((5000, 8, 8))
 
 
+def test_probes_reject_bad_arguments_before_device():
+    for args in [(0, 0, 1), (65, 1, 1), (5, 0, 0)]:
+        with pytest.raises(F.Fs3dError) as ei:
+            F.scatter_probe(*args)
+        assert ei.value.code == 1
+    with pytest.raises(F.Fs3dError) as ei:
+        F.copy_probe(15)
+    assert ei.value.code == 1
+
+
 def test_last_error_is_thread_local_message():
     with pytest.raises(F.Fs3dError) as ei:
         F.World((-1, 2, 2))
