@@ -720,7 +720,8 @@ def main():
             "dense_gvox_per_s": round(p["vox"] * p["ticks"] / pd["wall"] / 1e9, 2),
             "dense_kernel_ms": round(pd["kern_ms"], 4), "dense_achieved_GBps": round(ach2 / 1e9, 1),
             "dense_hbm_frac": round(frac2, 4), "dense_frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
-            "fused_pairs": {"kernel": "tick2_kernel<32> (two dense ticks per pass, first tick's rows in LDS)",
+            "fused_pairs": {"kernel": "tick2c_kernel<16> (two dense ticks per pass in 16-byte x-chunks of 1 KiB, "
+                                             "first tick's rows in LDS)",
                             "gvox_per_s": round(p["vox"] * p["ticks"] / p["fused"]["wall"] / 1e9, 2),
                             "ms_per_tick": round(p["fused"]["kern_ms"], 4),
                             "achieved_GBps": round(p["vox"] * 2 / (2 * p["fused"]["kern_ms"] * 1e-3) / 1e9, 1),

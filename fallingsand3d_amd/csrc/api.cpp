@@ -194,8 +194,8 @@ bool fuse_ok(const fs3d_world *w) { return !w->skip && w->fuse && tick2_supporte
 
 // Grids up to this many voxels keep both buffers within about twice the 256 MB
 // MALL, where the single-tick kernel is not HBM-bound and fusion's redundant
-// halo work costs more than the traffic it saves (512^3: 74.5 vs 46.5 us per
-// tick; 1024^3: 369 vs 402 us; 2048^3: 2.40 vs 3.22 ms).
+// halo work costs more than the traffic it saves (512^3: 48.4 vs 47.7 us per
+// tick; 1024^3: 324 vs 390 us; 2048^3: 2.40 vs 3.22 ms).
 constexpr u64 kFuseMinVoxels = 1ull << 28;
 
 // The fused pass's launch shape for rows up to 1024 bytes: the smallest segment
@@ -208,8 +208,11 @@ void plan_fused(fs3d_world *w) {
   w->fuse = (u64)w->nx * (u64)w->ny * (u64)w->nz > kFuseMinVoxels && tick2_supported(w->shape2, w->nx, nullptr);
 }
 
-// z row-quads per workgroup of the fused two-tick pass
-constexpr int kTick2ZLen = 32;
+// z row-quads per workgroup of the fused two-tick pass.  Eight keep enough
+// workgroups in flight on smaller grids (us per fused tick, z = 4 / 8 / 16 /
+// 32: 512^3 47.3 / 48.4 / 49.2 / 69.5; 1024^3 336 / 324 / 326 / 341; 2048^3
+// 2530 / 2400 / 2400 / 2400); the z halo costs 1/8 more tick-A rows.
+constexpr int kTick2ZLen = 8;
 
 // Dense single ticks of rows longer than 1024 bytes (with the automatic
 // segment) run one wave per 16-byte x-chunk of 1 KiB instead of one wave per

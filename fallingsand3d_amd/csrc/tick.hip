@@ -36,6 +36,12 @@ struct TickArgs {
 
 __device__ __forceinline__ u64 tick_of(const TickArgs &a) { return a.tick_dev ? *a.tick_dev + a.k_off : a.t; }
 
+// Sector size of the full-grid skipping tick's load and store decisions: a
+// partly written 64-byte sector costs more than a whole one (512^3 box scene,
+// ticks 5-25 with the last hashed: 50.1 against 52.0 us per tick with
+// segment-sized decisions; ticks 100-300: 41.2 against 42.2).
+constexpr int kSkipSectorBytes = 64;
+
 // The grid is x: groups of 4 * rq_per_wave row quads along y (one group per
 // workgroup), y and z: z row quads (zq = blockIdx.z * 65535 + blockIdx.y), so a
 // lane's row quad comes from the block indices without a division.
@@ -77,7 +83,20 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   R.zq = active ? zq : 0;
   R.q = (long long)R.zq * nyq + R.yq;
   R.skip = false;
-  if constexpr (SKIP) rq_skip_check(R, p, &sk);
+  if constexpr (SKIP) {
+    rq_skip_check(R, p, &sk);
+    // A settled segment whose 64-byte sector holds an active one is evaluated
+    // too (it keeps its bytes), so that stores cover whole sectors.
+    if constexpr (S < kSkipSectorBytes) {
+#pragma unroll
+      for (int d = 1; d < kSkipSectorBytes / S; d <<= 1) {
+        const int o = __shfl_xor((int)R.skip, d, 64);
+        const int ol = lane ^ d;
+        const bool same = (int)(((u32)ol * a.lr_magic) >> 16) == rql;  // partner in the same row quad
+        R.skip = R.skip && (!same || o != 0);
+      }
+    }
+  }
   rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, p, !SKIP || !R.skip || sk.load_skipped);
   if constexpr (SKIP && !HASH) {
     if (__ballot(R.active && !R.skip) == 0) {  // the whole wave is settled
