@@ -83,8 +83,10 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
   R.zq = active ? zq : 0;
   R.q = (long long)R.zq * nyq + R.yq;
   R.skip = false;
+  bool rule_skip = false;  // settled by the skip rule (before sector merging)
   if constexpr (SKIP) {
     rq_skip_check(R, p, &sk);
+    rule_skip = R.skip;
     // A settled segment whose 64-byte sector holds an active one is evaluated
     // too (it keeps its bytes), so that stores cover whole sectors.
     if constexpr (S < kSkipSectorBytes) {
@@ -119,9 +121,12 @@ __global__ __launch_bounds__(256) void tick_kernel(TickArgs a) {
     }
     if (a.stats) {
       const u64 vox = R.active ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
-      const u64 swept = wave_sum_u64(R.skip ? 0 : vox);
-      const u64 ronly = wave_sum_u64(R.skip && HASH ? vox : 0);
-      const u64 bad = __ballot(R.skip && ch);  // a settled segment changed: must never happen
+      // swept: the segments the skip rule evaluates (sector merging adds
+      // traffic, not work the rule needs); read-only: settled segments read
+      // for the hash
+      const u64 swept = wave_sum_u64(rule_skip ? 0 : vox);
+      const u64 ronly = wave_sum_u64(rule_skip && HASH ? vox : 0);
+      const u64 bad = __ballot(rule_skip && ch);  // a settled segment changed: must never happen
       if (lane == 0) {
         u64 *slot = a.stats + (size_t)(wave % kHashParts) * kHashStride;
         atomicAdd((unsigned long long *)slot, (unsigned long long)swept);
@@ -488,7 +493,11 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
         acc_e |= run_e;
       }
       if (a.stats) {
-        const u64 vox = lv ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
+        // the listed segments only (a sector's settled segments add traffic,
+        // not work the skip rule needs)
+        const u64 aw = ((u64)(u32)__shfl((int)(act >> 32), o, 64) << 32) | (u32)__shfl((int)(u32)act, o, 64);
+        const bool listed = lv && ((aw >> seg) & 1ull);
+        const u64 vox = listed ? (u64)R.m * (u64)__builtin_popcount(R.valid) : 0;
         const u64 swept = wave_sum_u64(vox);
         if (lane == 0 && swept)
           atomicAdd((unsigned long long *)(a.stats + (size_t)(wave % kHashParts) * kHashStride),

@@ -139,10 +139,12 @@ int fs3d_world_get_skip(const fs3d_world *w, int32_t *mode);
 int fs3d_world_set_fuse(fs3d_world *w, int32_t on);
 int fs3d_world_get_fuse(const fs3d_world *w, int32_t *on);
 /* Sweep counters of the skipping tick: enable != 0 starts (or keeps) counting,
- * 0 stops.  Returns and clears what was counted so far: out[0] = voxels read and
- * written, out[1] = voxels only read (settled segments on hash ticks), out[2] =
- * settled segments whose blocks changed on a hash tick (always 0 unless the
- * skipping bookkeeping is wrong).  Synchronises. */
+ * 0 stops.  Returns and clears what was counted so far: out[0] = voxels of the
+ * segments the skip rule updates (the whole-sector loads and stores around them
+ * are not counted), out[1] = voxels only read (settled segments on hash ticks),
+ * out[2] = settled segments whose blocks changed where the full-grid schedule
+ * evaluated them (hash ticks, and beside an active segment of their 64-byte
+ * sector; always 0 unless the skipping bookkeeping is wrong).  Synchronises. */
 int fs3d_world_sweep_stats(fs3d_world *w, int32_t enable, uint64_t out[3]);
 
 /* ---- scenes (seeded, bit-exact, generated by this library) ------------------- */
