@@ -466,9 +466,13 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch, &echg);
       // OR of the change bits of each entry's run of lanes in this group (a
       // neighbour's cell changed by this lane sets the neighbour's bit) ...
+      // A straddling block that changed sets the bit of a neighbour that is not
+      // being updated as well, even when only this lane's half of it moved:
+      // the full-grid skip rule (hash ticks) reads a segment's bit as covering
+      // every block it evaluates, the two straddling its edges included.
       u64 bits = (lv && ch) ? (1ull << seg) : 0ull;
-      if (echg & 1u) bits |= 1ull << (seg - 1);
-      if (echg & 2u) bits |= 1ull << (seg + 1);
+      if (p && R.sl && (echg & 5u)) bits |= 1ull << (seg - 1);
+      if (p && R.sr && (echg & 10u)) bits |= 1ull << (seg + 1);
       // edge bits: the lane's first block touches edge seg, its last edge seg + 1
       u64 ebits = 0;
       if (lv && (echg & 4u)) ebits |= 1ull << seg;

@@ -89,6 +89,69 @@ def test_skip_full_width_rows(gpu, seg, mode):
         assert w.sweep_stats(False)["violations"] == 0
 
 
+def _edge_scene(rng, shape, seg):
+    """A settled base (static floor, a sand bed, static posts) with isolated sand
+    grains and fluid drops next to segment and 64-byte sector edges: most
+    segments are settled, and the active ones sit beside settled neighbours, so
+    on odd ticks the blocks straddling their edges are mostly fixed points."""
+    nx, ny, nz = shape
+    g = np.zeros((nz, ny, nx), np.uint8)
+    g[0] = 1
+    g[1:3] = 2
+    g[1:nz // 2, ::5, ::97] = 1
+    edges = [b + d for b in range(seg, nx, seg) for d in (-2, -1, 0, 1)]
+    for x in edges:
+        if rng.random() < 0.6:
+            y, z = int(rng.integers(0, ny)), int(rng.integers(nz // 3, nz))
+            g[z, y, x] = 2 if rng.random() < 0.7 else 3
+    # a few busy row quads whose active sectors spill across groups of items
+    zb = nz - 3
+    g[zb:zb + 2, 0:2, :] = np.where(rng.random((2, min(2, ny), nx)) < 0.4, 2, 0)
+    return g
+
+
+@pytest.mark.parametrize("shape,seg", [((2048, 6, 26), 32), ((1000, 8, 22), 16), ((2000, 5, 30), 32),
+                                       ((512, 9, 24), 8)],
+                         ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+@MODES
+def test_skip_sparse_activity_at_edges(gpu, O, shape, seg, mode):
+    # Sparse activity beside settled segments exercises the schedules' settled
+    # neighbours: the full-grid tick's keep-input blocks and whole-sector stores,
+    # and the packed tick's sector items and edge masks (a straddling block known
+    # not to change keeps its input), across hash ticks, edits and tick resets.
+    rng = np.random.default_rng(shape[0] + seg)
+    g = _edge_scene(rng, shape, seg)
+    t0 = int(rng.integers(0, 1 << 16)) | 1  # starts on an odd tick
+    announce(f"skip vs dense, sparse edge activity {shape} seg={seg} t0={t0} mode={mode}")
+    with gpu.World(shape) as w, gpu.World(shape, flags=gpu.DENSE) as d:
+        w.skip = mode
+        for x in (w, d):
+            x.segment = seg
+            x.upload(g)
+            x.tick_count = t0
+        w.sweep_stats(True)
+        cur, t = g.copy(), t0
+        for k in range(8):
+            assert [int(v) for v in w.run(15, 7)] == [int(v) for v in d.run(15, 7)], k
+            cur, _ = O.run(cur, t, 15, use_lut=True)
+            t += 15
+            if k in (2, 5):
+                # edits on both sides of an edge, while both worlds run on
+                for x, y, z, v in ((seg, 1, shape[2] - 2, 3), (seg - 1, 2, shape[2] - 3, 2),
+                                   (2 * seg + 1, 0, shape[2] - 1, 2)):
+                    for wd in (w, d):
+                        wd.set_voxel(x, y, z, v)
+                    cur[z, y, x] = v
+            if k == 6:
+                t += 3
+                for wd in (w, d):
+                    wd.tick_count = t
+        a = w.download()
+        assert (a == d.download()).all()
+        assert (a == cur).all()
+        assert w.sweep_stats(False)["violations"] == 0
+
+
 @MODES
 def test_skip_with_host_edits(gpu, O, mode):
     shape = (72, 30, 40)
