@@ -577,19 +577,54 @@ __global__ __launch_bounds__(256) void tick_chunk_kernel(TickArgs a, int chunks)
   }
 }
 
+// Full-grid state hash.  A wave takes 1 KiB (16 bytes per lane), compacts the
+// non-empty cells into LDS (one 32-bit item per cell: value << 16 | byte offset
+// within the 1 KiB) and mixes them 64 at a time, so each cell costs a mix only
+// when it is non-empty; mixing under per-byte branches would cost the wave a
+// mix for every byte position that any of its lanes has non-empty.
 __global__ __launch_bounds__(256) void hash_kernel(const u8 *g, size_t n, u64 *parts) {
+  __shared__ u32 items[4][1024];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32 *it = items[wv];
   u64 acc = 0;
   const size_t nchunk = (n + 15) / 16;
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunk;
-       c += (size_t)gridDim.x * blockDim.x) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // block-uniform loop (two barriers per pass); c0 is this wave's first chunk
+  for (size_t cb = (size_t)blockIdx.x * blockDim.x; cb < nchunk; cb += stride) {
+    const size_t c0 = cb + (size_t)wv * 64, c = c0 + lane;
     const size_t base = c * 16;
-    const u32x4 v = *reinterpret_cast<const u32x4 *>(g + base);  // buffers are 256-byte aligned
-    const u32 d[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const u32 b = (d[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-      if (b && base + k < n) acc += dmix64(((u64)(base + k) << 8) | b);
+    u32 d[4] = {0u, 0u, 0u, 0u};
+    if (c < nchunk) {
+      const u32x4 v = *reinterpret_cast<const u32x4 *>(g + base);  // buffers are 256-byte aligned
+      d[0] = v.x;
+      d[1] = v.y;
+      d[2] = v.z;
+      d[3] = v.w;
     }
+    u32 nz = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      nz |= (((d[k >> 2] >> (8 * (k & 3))) & 0xFFu) != 0u && base + k < n) ? (1u << k) : 0u;
+    const int cnt = __builtin_popcount(nz);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int excl = incl - cnt;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      if ((nz >> k) & 1u)
+        it[excl + __builtin_popcount(nz & ((1u << k) - 1u))] =
+            (((d[k >> 2] >> (8 * (k & 3))) & 0xFFu) << 16) | (u32)(lane * 16 + k);
+    __syncthreads();
+    for (int j = lane; j < total; j += 64) {
+      const u32 x = it[j];
+      acc += dmix64(((u64)(c0 * 16 + (x & 0x3FFu)) << 8) | (x >> 16));
+    }
+    __syncthreads();
   }
   __shared__ u64 wsum[4];
   acc = wave_sum_u64(acc);
@@ -602,7 +637,6 @@ __global__ __launch_bounds__(256) void hash_kernel(const u8 *g, size_t n, u64 *p
   }
 }
 
-// Sums the partial slots into *out and clears them for the next hash.
 __global__ __launch_bounds__(256) void hash_finalize_kernel(u64 *parts, u64 *out) {
   u64 acc = 0;
   for (int i = threadIdx.x; i < kHashParts; i += 256) {
