@@ -222,15 +222,30 @@ constexpr int kTick2ZLen = 8;
 // which packs several row quads into a wave.  0: the row-quad kernel.
 int dense_chunk_seg(const fs3d_world *w) { return w->seg_req == 0 && w->nx > 1024 ? 16 : 0; }
 
+// Whole-row-quad ticks of grids up to kInPlaceMaxBytes update in place: every
+// wave loads its row quads before it stores any of them, and the row quads of
+// one tick share no rows.  One buffer instead of two keeps a 512^3 world within
+// the 256 MB last-level cache (box scene, us per tick in place / from one buffer
+// into the other: ticks 5-25 with the last hashed 47.4 / 50.4; ticks 100-1100
+// 37.3 / 37.1; dense 42.9 / 47.3).  Larger grids live in HBM either way, where
+// reading and writing the same lines is slower (1024^3 dense: 405.7 / 388.9).
+// The packed schedule reads neighbouring segments' edge bytes from memory, and
+// the chunked and fused dense kernels neighbouring chunks' and tiles' rows, so
+// they always run from one buffer into the other.  A skipping world keeps one
+// mode for all its ticks (in place unless packed): a skipped segment relies on
+// the destination already holding its bytes.
+constexpr size_t kInPlaceMaxBytes = size_t(1) << 27;
+
 // Enqueues nticks ticks starting from buffer w->cur; *final receives the index of
 // the buffer holding the result (a fused pair of dense ticks moves the state to
-// the other buffer once, a single tick once).  With launch = false it only
-// computes *final.
+// the other buffer once, a single tick from one buffer into the other once, a
+// tick in place not at all).  With launch = false it only computes *final.
 int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev, int *final, bool launch = true) {
   const u64 *rule = device_rule_table(w->dev);
   u64 *const *masks = w->skip ? w->d_masks : nullptr;
   const int sched = w->skip ? skip_schedule(w) : FS3D_SKIP_OFF;
   const bool fuse = fuse_ok(w);
+  const bool inplace = w->n <= kInPlaceMaxBytes && !(masks && sched == FS3D_SKIP_SEGS);
   auto hashed_after = [&](int k) { return hash_every > 0 && (k + 1) % hash_every == 0; };
   int src = w->cur;
   for (int k = 0; k < nticks;) {
@@ -243,23 +258,26 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
       continue;
     }
     const bool hashed = hashed_after(k);
+    const bool chunked = !masks && dense_chunk_seg(w);
+    const bool packed = masks && sched == FS3D_SKIP_SEGS && !hashed;
+    const int dst = inplace && !chunked && !packed ? src : src ^ 1;
     if (launch) {
       if (masks && sched == FS3D_SKIP_LISTED && !hashed) {
         // settled row quads are listed out; hash ticks read every segment anyway
-        FS3D_HIP(launch_tick_listed(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+        FS3D_HIP(launch_tick_listed(w->shape, w->buf[src], w->buf[dst], w->nx, w->ny, w->nz,
                                     w->tick + (u64)k, tick_dev, (u32)k, rule, w->stream, masks,
                                     w->d_stats, w->d_list, w->d_ctl, w->mask_words, w->list_waves));
-      } else if (masks && sched == FS3D_SKIP_SEGS && !hashed) {
-        FS3D_HIP(launch_tick_packed(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz, w->tick + (u64)k,
+      } else if (packed) {
+        FS3D_HIP(launch_tick_packed(w->shape, w->buf[src], w->buf[dst], w->nx, w->ny, w->nz, w->tick + (u64)k,
                                     tick_dev, (u32)k, rule, w->stream, masks, w->d_stats, w->d_list, w->d_ctl,
                                     w->mask_words, w->pack_waves, kPackChunk));
       } else {
-        if (!masks && dense_chunk_seg(w))
-          FS3D_HIP(launch_tick_chunked(dense_chunk_seg(w), w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+        if (chunked)
+          FS3D_HIP(launch_tick_chunked(dense_chunk_seg(w), w->buf[src], w->buf[dst], w->nx, w->ny, w->nz,
                                        w->tick + (u64)k, tick_dev, (u32)k, rule, hashed ? w->d_parts : nullptr,
                                        w->stream));
         else
-          FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz,
+          FS3D_HIP(launch_tick(w->shape, w->buf[src], w->buf[dst], w->nx, w->ny, w->nz,
                                w->tick + (u64)k, tick_dev, (u32)k, rule,
                                hashed ? w->d_parts : nullptr, w->stream, masks, w->d_stats, w->d_ctl));
         if (hashed) FS3D_HIP(launch_hash_finalize(w->d_parts, w->d_small + (k + 1) / hash_every, w->stream));
@@ -269,7 +287,7 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
         FS3D_HIP(launch_occ_dirty(masks, w->tick + (u64)k, tick_dev, (u32)k, w->nx, w->ny, w->nz, w->shape.seg,
                                   w->d_dirty, w->stream));
     }
-    src ^= 1;
+    src = dst;
     k++;
   }
   if (final) *final = src;
@@ -720,9 +738,9 @@ int fs3d_world_load(fs3d_world *w, const char *path) {
   FS3D_HIP(hipSetDevice(w->dev));
   // The payload goes to the idle buffer and becomes the world's state only once
   // its hash matches the stored one; on any failure the grid and tick stay as
-  // they were.  The idle buffer holds the previous tick's output, which skipped
-  // segments rely on, so every change bit is set either way (mark_all): the
-  // next two ticks evaluate every segment and rewrite both buffers.
+  // they were.  Skipped segments rely on the destination buffer holding their
+  // bytes, so every change bit is set either way (mark_all): the next two ticks
+  // evaluate every segment and rewrite whichever buffers they write.
   int rc = file_read(path, nullptr, w->n, load_chunk, w);
   u64 h = 0;
   if (!rc) {
