@@ -161,8 +161,9 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
 // barrier; in both, the straddling block is evaluated by both end lanes from
 // the same inputs and each stores its own bytes.  Wave w takes row quad w /
 // chunks and chunk w % chunks.
-template <int S>
+template <int S, int TY>
 __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
+  constexpr int kTYc = TY, kWavesC = TY + 1, kLayerRowsC = 2 * TY;
   extern __shared__ __attribute__((aligned(16))) u8 lds[];
   constexpr int ND = S / 4;
   const int lane = threadIdx.x & 63;
@@ -177,16 +178,16 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
   const u32 xcd = bid % kXcds, per = G / kXcds, extra = G % kXcds;
   const int tile = (int)(xcd * per + min(xcd, extra) + bid / kXcds);
   const int ytile = tile % a.ytiles, zseg = tile / a.ytiles;
-  const int Y0 = ytile * kTY;
+  const int Y0 = ytile * kTYc;
   const int YA0 = Y0 - 1 + pA;
   const int y_lo = 2 * YA0 - pA + 1;
   const int Z0 = zseg * a.zlen;
   const int Z1 = min(Z0 + a.zlen, nzqB);
   const int LR = a.lanes_per_row;  // S-byte segments per row
-  const size_t layer_bytes = (size_t)kLayerRows * a.stride;
+  const size_t layer_bytes = (size_t)kLayerRowsC * a.stride;
   const int s_first = Z0 + pA - 1, s_last = Z1 - 1 + pA;
   const int yqA = YA0 + wv;
-  const int unused = (wv == 0 ? 0x5 : 0) | (wv == kWaves - 1 ? 0xA : 0);
+  const int unused = (wv == 0 ? 0x5 : 0) | (wv == kWavesC - 1 ? 0xA : 0);
 
   // tick A's row quad (yqA, s), chunk c, fetched one step ahead of its update;
   // the chunk ends' edge bytes come from HBM on odd ticks
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
     __syncthreads();
     // ---- tick B: row quad (Y0 + wv, s - pA), chunk c, from LDS into HBM ----
     const int b = s - pA;
-    if (wv < kTY && b >= Z0 && b < Z1) {  // uniform per wave; no barrier inside
+    if (wv < kTYc && b >= Z0 && b < Z1) {  // uniform per wave; no barrier inside
       RowQuad<S> Q;
       Q.seg = seg;
       Q.yq = Y0 + wv;
@@ -290,10 +291,18 @@ constexpr int kTick2ChunkSeg = 16;
 
 size_t tick2c_stride(int nx) { return ((size_t)nx + 15) & ~(size_t)15; }
 
+// Tick-B row quads per workgroup along y of the chunked pass: 5 where the LDS
+// ring (3 layers x 10 rows) fits 64 KiB and a workgroup of 6 x chunks waves fits
+// 1024 threads, else 3.  Deeper tiles recompute fewer tick-A halo rows.
+int tick2c_ty(int nx) {
+  const int chunks = (int)((tick2c_stride(nx) / kTick2ChunkSeg + 63) / 64);
+  return 3 * 10 * tick2c_stride(nx) <= 64 * 1024 && 6 * chunks <= 16 ? 5 : 3;
+}
+
 bool tick2_supported(const TickShape &sh, int nx, size_t *lds_bytes) {
   size_t bytes = 0;
   if (nx > kTick2ChunkMinNx) {
-    bytes = 3 * (size_t)kLayerRows * tick2c_stride(nx);
+    bytes = 3 * 2 * (size_t)tick2c_ty(nx) * tick2c_stride(nx);
   } else {
     if (sh.rq_per_wave != 1) return false;
     bytes = 3 * (size_t)kLayerRows * tick2_stride(sh);
@@ -317,7 +326,8 @@ hipError_t launch_tick2(const TickShape &sh, const u8 *src, u8 *dst, int nx, int
   // graph replays read the tick from the device: size the tiles for the larger
   // parity's row-quad counts (the kernel bounds itself by tick B's)
   const int nyq_max = (ny + 2) >> 1, nzq_max = (nz + 2) >> 1;
-  a.ytiles = (nyq_max + kTY - 1) / kTY;
+  const int ty = nx > kTick2ChunkMinNx ? tick2c_ty(nx) : kTY;
+  a.ytiles = (nyq_max + ty - 1) / ty;
   a.zlen = zlen;
   a.t = t;
   a.k_off = k_off;
@@ -329,7 +339,10 @@ hipError_t launch_tick2(const TickShape &sh, const u8 *src, u8 *dst, int nx, int
     a.lanes_per_row = (nx + S - 1) / S;
     a.stride = (int)tick2c_stride(nx);
     const int chunks = (a.lanes_per_row + 63) / 64;  // 2 at nx = 2048, at most 4 within the LDS limit
-    hipLaunchKernelGGL((tick2c_kernel<S>), dim3((unsigned)blocks), dim3(kWaves * chunks * 64), lds, s, a, chunks);
+    if (ty == 5)
+      hipLaunchKernelGGL((tick2c_kernel<S, 5>), dim3((unsigned)blocks), dim3(6 * chunks * 64), lds, s, a, chunks);
+    else
+      hipLaunchKernelGGL((tick2c_kernel<S, 3>), dim3((unsigned)blocks), dim3(4 * chunks * 64), lds, s, a, chunks);
     return hipGetLastError();
   }
   const dim3 grid((unsigned)blocks), block(kWaves * 64);
