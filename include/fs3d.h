@@ -116,12 +116,14 @@ int fs3d_world_get_segment(const fs3d_world *w, int32_t *seg_bytes);
  * (whether a block is a fixed point does not depend on its variant).  Three
  * schedules: LISTED runs a plan pass that lists the row quads with work and a
  * persistent pass over the list (settled regions cost nothing); SEGS reads the
- * same list and packs the listed row quads' active lane segments 64 to a wave
- * (the block update is paid per active segment, not per row quad with one);
- * WAVES launches one wave per row-quad group that skips on its own (lower
+ * same list and packs the listed row quads' active 64-byte sectors into full
+ * waves (the block update is paid per active sector, not per row quad with
+ * one); WAVES launches one wave per row-quad group that skips on its own (lower
  * per-tick overhead; best for small grids).  AUTO picks SEGS above 2^17 waves
  * of work per tick, else WAVES.  Hash ticks read settled segments in all
- * three.
+ * three.  Worlds of up to 2^27 voxels tick in place with WAVES and LISTED (and
+ * dense single ticks of rows up to 1024 bytes); the grid stays where
+ * fs3d_world_device_ptr last pointed only in that case.
  * Every host-side edit (set_voxel, upload, fills, set_tick, set_segment) keeps
  * the bookkeeping exact; writing the grid through fs3d_world_device_ptr is not
  * supported. */
