@@ -428,13 +428,12 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       R.zq = (int)zq;
       R.q = (long long)zq * nyq + yq;
       R.skip = false;
-      // No keep-input: on odd ticks a straddling block always sees its true
-      // bytes, from the adjacent lane when it holds the neighbour segment and
-      // from memory otherwise; next to a sector that is not listed this lane
-      // also stores the neighbour's cell of that block.
-      // A straddling block that is known not to change (its edge bit is clear
-      // on an odd tick) keeps its input instead: nothing is read from the
-      // neighbour, whose line then often stays unfetched.
+      // On odd ticks a straddling block sees its true bytes: from the adjacent
+      // lane when it holds the neighbour segment, from memory otherwise (and
+      // next to a segment that is not being updated this lane also stores the
+      // neighbour's cell of that block).  A straddling block known not to
+      // change (its edge bit is clear) keeps its input instead: nothing is read
+      // from the neighbour, whose line then often stays unfetched.
       const bool lfix = p && !((eo >> seg) & 1ull);
       const bool rfix = p && seg < 63 && !((eo >> (seg + 1)) & 1ull);
       const bool lmem = lv && seg > 0 && (lane == 0 || !left_act);
@@ -464,12 +463,11 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       bool ch = false;
       u32 echg = 0;
       rq_update<S, false, true, true, true>(R, a.dst, LR, t, a.rule, &ch, &echg);
-      // OR of the change bits of each entry's run of lanes in this group (a
-      // neighbour's cell changed by this lane sets the neighbour's bit) ...
+      // OR of the change bits of each entry's run of lanes in this group ...
       // A straddling block that changed sets the bit of a neighbour that is not
-      // being updated as well, even when only this lane's half of it moved:
-      // the full-grid skip rule (hash ticks) reads a segment's bit as covering
-      // every block it evaluates, the two straddling its edges included.
+      // being updated too, even when only this lane's half of it moved: the
+      // full-grid skip rule (hash ticks) reads a segment's bit as covering every
+      // block it evaluates, the two straddling its edges included.
       u64 bits = (lv && ch) ? (1ull << seg) : 0ull;
       if (p && R.sl && (echg & 5u)) bits |= 1ull << (seg - 1);
       if (p && R.sr && (echg & 10u)) bits |= 1ull << (seg + 1);
