@@ -392,9 +392,11 @@ def bench_copy(F, nbytes, iters=20, scatter=True):
     out["cache_resident"] = dict(nbytes=small, ms=round(ms_s, 5), GBps=round(2 * small / (ms_s * 1e-3) / 1e9, 1))
     log(f"[bench] copy, 128 MiB working set: {2 * small / ms_s / 1e6:.1f} GB/s")
     if scatter:
-        for sector in (False, True):
-            sms, sb = F.scatter_probe(5, sector)
-            key = "scatter5_sector64" if sector else "scatter5_seg32"
+        # 5 of the 64 segments: the packed tick's shape; all 64: the dense tick's
+        # (every row quad's four rows, one 32-byte segment of each per lane)
+        for key, per_rq, sector in (("scatter5_seg32", 5, False), ("scatter5_sector64", 5, True),
+                                    ("rowquads_all", 64, False)):
+            sms, sb = F.scatter_probe(per_rq, sector)
             out[key] = dict(ms=round(sms, 4), item_bytes=sb, GBps=round(sb / (sms * 1e-3) / 1e9, 1))
             log(f"[bench] {key}: {sb / 1e9:.3f} GB of items in {sms:.4f} ms")
     return out
@@ -698,6 +700,9 @@ def main():
                "kernel": DENSE2048_LABEL,
                "kernel_ms": round(p["dense"]["kern_ms"], 5), "alg_bytes_per_tick": 2 * p["vox"],
                "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
+        if c.get("rowquads_all"):
+            # against the same access shape without arithmetic (every row quad's four rows)
+            hbm["frac_of_access_shape"] = round(ach2 / 1e9 / c["rowquads_all"]["GBps"], 4)
         if traffic2048 is not None:
             if "error" in traffic2048:
                 hbm["traffic_error"] = traffic2048["error"]
@@ -741,7 +746,7 @@ def main():
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4),
                          "scatter5_seg32": c.get("scatter5_seg32"), "scatter5_sector64": c.get("scatter5_sector64"),
-                         "cache_resident": c.get("cache_resident")}
+                         "cache_resident": c.get("cache_resident"), "rowquads_all": c.get("rowquads_all")}
         bb = bench_batched(F)
         extra["batched"] = {
             "workload": "4096 worlds, edges log-uniform 16-128, rain fill, density U(0.05,0.5), 500 ticks",
