@@ -275,3 +275,29 @@ def test_skip_reserved_bytes_on_segment_edges(gpu, O, seg, mode):
         assert (out == want).all()
         assert ((out == 200) == (g == 200)).all() and ((out == 77) == (g == 77)).all()
         assert w.sweep_stats(False)["violations"] == 0
+
+
+@MODES
+def test_skip_from_one_buffer_into_the_other(gpu, mode):
+    # Grids above 2^27 bytes keep two buffers (smaller ones tick in place): the
+    # full-grid and listed schedules then rely on the destination holding the
+    # bytes of two ticks before.  1024 x 520 x 260 is 138 MB, just above the limit.
+    shape = (1024, 520, 260)
+    assert shape[0] * shape[1] * shape[2] > 1 << 27
+    announce(f"skip vs dense from one buffer into the other {shape} mode={mode}")
+    rng = np.random.default_rng(mode)
+    with gpu.World(shape) as w, gpu.World(shape, flags=gpu.DENSE) as d:
+        w.skip = mode
+        d.fuse = False
+        for x in (w, d):
+            x.fill_boxes(3)
+        w.sweep_stats(True)
+        for k in range(4):
+            assert [int(v) for v in w.run(12, 4)] == [int(v) for v in d.run(12, 4)], k
+            if k == 1:
+                for _ in range(20):
+                    x, y, z = int(rng.integers(0, shape[0])), int(rng.integers(0, shape[1])), int(rng.integers(1, shape[2]))
+                    for wd in (w, d):
+                        wd.set_voxel(x, y, z, 2)
+        assert w.hash() == d.hash()
+        assert w.sweep_stats(False)["violations"] == 0
