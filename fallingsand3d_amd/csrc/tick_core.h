@@ -155,10 +155,13 @@ struct SkipCtx {
 
 // One lane's share of one row quad: its S-byte segment of the 4 rows (r = dy +
 // 2*dz), loaded, plus where it lives.  Out-of-grid rows read as STATIC.
-template <int S>
+// Off is the type of the rows' byte offsets: u32 for grids of at most 2^32
+// bytes (one register per address, a 32-bit offset from the uniform base
+// pointer), else size_t.
+template <int S, class Off = size_t>
 struct RowQuad {
   u32 own[4][S / 4];
-  size_t rowoff[4];
+  Off rowoff[4];
   long long q;  // row-quad index within the tick
   int valid;    // bit r: row r is inside the grid
   int m;        // valid bytes of the segment
@@ -177,8 +180,9 @@ struct RowQuad {
 
 // Loads the lane's segment of the 4 rows of row quad (R.yq, R.zq) at parity p
 // (R.seg and R.active set); fetch = false leaves placeholders (STATIC).
-template <int S>
-__device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, int ny, int nz, int p, bool fetch) {
+template <int S, class Off>
+__device__ __forceinline__ void rq_fetch(RowQuad<S, Off> &R, const u8 *src, int nx, int ny, int nz, int p,
+                                         bool fetch) {
   constexpr int ND = S / 4;
   R.loaded = fetch;
   const int y0 = 2 * R.yq - p, z0 = 2 * R.zq - p;
@@ -190,7 +194,7 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, i
     const int y = y0 + (r & 1), z = z0 + (r >> 1);
     const bool ok = R.active && y >= 0 && y < ny && z >= 0 && z < nz;
     R.valid |= ok ? (1 << r) : 0;
-    R.rowoff[r] = ((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs;
+    R.rowoff[r] = (Off)(((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs);
 #pragma unroll
     for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
     if (ok && fetch) load_seg<S>(src + R.rowoff[r], R.own[r]);
@@ -210,8 +214,8 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S> &R, const u8 *src, int nx, i
 // The five mask loads are issued unconditionally (out-of-range ones from word 0,
 // then discarded) so that they overlap: loads under per-lane branches are each
 // waited for before the next is issued.
-template <int S>
-__device__ __forceinline__ void rq_skip_check(RowQuad<S> &R, int p, const SkipCtx *sk) {
+template <int S, class Off>
+__device__ __forceinline__ void rq_skip_check(RowQuad<S, Off> &R, int p, const SkipCtx *sk) {
   u64 v[4];
   bool in[4];
 #pragma unroll
@@ -267,9 +271,9 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // left / right neighbour's cell (R.sl / R.sr) because it changed, and bit 2 /
 // bit 3 when the lane's first / last block changed (on odd ticks the blocks
 // straddling its left / right edge).
-template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P>
-__device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
-                                           bool *changed, u32 *edge_changed) {
+template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P, class Off>
+__device__ __forceinline__ u64 rq_update_p(const RowQuad<S, Off> &R, u8 *dst, int LR, u32 t,
+                                           const u64 *__restrict__ rule, bool *changed, u32 *edge_changed) {
   constexpr int ND = S / 4;  // dwords per lane per row
   const int p = P < 0 ? (int)(t & 1u) : P;
   const int seg = R.seg;
@@ -476,7 +480,7 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
             for (int kk = 0; kk < 4; kk++) {
               const int k = 4 * j + kk;
               const u32 v = (res[j] >> (8 * kk)) & 0xFFu;
-              if (k < R.m && v) acc += dmix64(((u64)(R.rowoff[r] + k) << 8) | v);
+              if (k < R.m && v) acc += dmix64((((u64)R.rowoff[r] + (u64)k) << 8) | v);
             }
           }
         }
@@ -489,9 +493,10 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S> &R, u8 *dst, int LR,
 // One copy of the update per parity, chosen by a branch that is uniform: the
 // tick's parity is the same across a wave in every caller (kernel arguments, or
 // one world per workgroup).
-template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false>
-__device__ __forceinline__ u64 rq_update(const RowQuad<S> &R, u8 *dst, int LR, u32 t, const u64 *__restrict__ rule,
-                                         bool *changed = nullptr, u32 *edge_changed = nullptr) {
+template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false, class Off = size_t>
+__device__ __forceinline__ u64 rq_update(const RowQuad<S, Off> &R, u8 *dst, int LR, u32 t,
+                                         const u64 *__restrict__ rule, bool *changed = nullptr,
+                                         u32 *edge_changed = nullptr) {
   if (t & 1u) return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 1>(R, dst, LR, t, rule, changed, edge_changed);
   return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 0>(R, dst, LR, t, rule, changed, edge_changed);
 }
