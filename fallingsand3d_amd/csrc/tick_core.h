@@ -35,6 +35,8 @@ typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 // natural alignment, so segments are moved with unaligned vector accesses.
 typedef u32x2 __attribute__((aligned(1))) u32x2_u;
 typedef u32x4 __attribute__((aligned(1))) u32x4_u;
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+typedef u64x2 __attribute__((aligned(8))) u64x2_a8;  // two adjacent mask words
 
 constexpr u32 kStatic4 = 0x01010101u;
 
@@ -211,23 +213,40 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S, Off> &R, const u8 *src, int 
 
 // Sets R.skip for an active lane from the change masks (R.seg, R.yq, R.zq, R.q
 // set).
-// The five mask loads are issued unconditionally (out-of-range ones from word 0,
-// then discarded) so that they overlap: loads under per-lane branches are each
-// waited for before the next is issued.
+// The three mask loads (the own word, and per other-parity z row the two
+// adjacent words y = yq - p, yq + 1 - p as one 16-byte load) are issued
+// unconditionally (out-of-range ones from word 0, then discarded) so that they
+// overlap: loads under per-lane branches are each waited for before the next is
+// issued.  A pair read past the last word of a parity's row quads reads its
+// edge words, which follow in the same allocation (512^3 box scene: 45.3
+// against 46.1 us per tick over ticks 5-25 with the last hashed, 35.8 against
+// 36.7 over ticks 100-1100, with five 8-byte loads).
 template <int S, class Off>
 __device__ __forceinline__ void rq_skip_check(RowQuad<S, Off> &R, int p, const SkipCtx *sk) {
-  u64 v[4];
-  bool in[4];
+  u64x2 v[2];
+  bool in[4];  // word (y = yq + (k & 1) - p, z = zq + (k >> 1) - p) is a row quad
+  long long i0[2];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int yo = R.yq + (k & 1) - p, zo = R.zq + (k >> 1) - p;
-    in[k] = R.active && yo >= 0 && yo < sk->nyq_o && zo >= 0 && zo < sk->nzq_o;
-    v[k] = sk->other[in[k] ? (long long)zo * sk->nyq_o + yo : 0];
+  for (int kz = 0; kz < 2; kz++) {
+    const int zo = R.zq + kz - p;
+    const bool zin = R.active && zo >= 0 && zo < sk->nzq_o;
+#pragma unroll
+    for (int ky = 0; ky < 2; ky++) {
+      const int yo = R.yq + ky - p;
+      in[2 * kz + ky] = zin && yo >= 0 && yo < sk->nyq_o;
+    }
+    // a pair that would start before word 0 (y = -1 in row 0) is read from
+    // word 0, which is then its second word
+    i0[kz] = zin ? (long long)zo * sk->nyq_o + R.yq - p : 0;
+    v[kz] = *reinterpret_cast<const u64x2_a8 *>(sk->other + (i0[kz] < 0 ? 0 : i0[kz]));
   }
   const u64 mine = sk->mine[R.active ? R.q : 0];
   u64 o = 0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) o |= in[k] ? v[k] : 0ull;
+  for (int kz = 0; kz < 2; kz++) {
+    const u64 w0 = v[kz].x, w1 = i0[kz] < 0 ? v[kz].x : v[kz].y;
+    o |= (in[2 * kz] ? w0 : 0ull) | (in[2 * kz + 1] ? w1 : 0ull);
+  }
   // Odd ticks: segments seg, seg+1 and seg-1 at tick t-1 (the straddling
   // blocks read a byte of each neighbour; shifted right only: seg may be 63).
   // Even ticks: segment seg alone.  No block straddles a segment edge at even
