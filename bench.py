@@ -24,13 +24,15 @@ Default run (one JSON line on stdout):
            every 10 ticks; copy: a plain device copy of the same bytes, the
            practical bandwidth ceiling; batched: 4096 worlds against the
            one-launch-per-world loop; serving: the continuous-batching server
-           under a Poisson load, saturated, and against one world per request.
+           under a Poisson load, saturated, and against one world per request;
+           slab2048: one GPU's z-slab of the 2048^3 world (dense and fused ticks
+           with a two-plane halo each side) and its halo exchange.
            Each workload's hashes (and frame digests) are compared with the
            oracle's fixtures in tests/golden/.
   cpu_baseline  the oracle's tick on this host, one thread, a bounded sample; its
            "openmp" field times the same tick over the host threads the run may use.
 
---only {boxes512,pillars2048,copy,batched,serving,render2048} runs one workload
+--only {boxes512,pillars2048,copy,batched,serving,render2048,slab2048} runs one workload
 (for profiler runs) and prints its own JSON line; --only boxes512-window and
 --only dense2048-window run just a timed window (and a copy of known size), for
 the counter runs.
@@ -45,6 +47,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+XGMI_LINK_BPS = 153e9  # one MI355X xGMI link, per direction (published figure; one GPU here)
 BYTES_PER_UPDATE = 2
 HASH_EVERY = 50
 COPY_CAL_BYTES = 1 << 30  # the known-size copy that scales the byte counters
@@ -82,10 +85,12 @@ def hash_segments(steps, every=HASH_EVERY):
 def timed_boxes_window(w, warmup, steps):
     """Warm-up ticks, then the timed graph runs of hash_segments(steps).  Both
     graphs are captured beforehand: the first run has an even tick count, so the
-    second starts from the same buffer it was captured on.  Returns (wall s,
-    device ms per tick, hashes, absolute tick of each hash)."""
+    second starts from the same buffer it was captured on.  The warm-up hashes
+    its last tick, so the hashing kernels' first use in the process (about 50 us
+    of one-time setup) falls outside the timed region.  Returns (wall s, device ms
+    per tick, hashes, absolute tick of each hash)."""
     if warmup:
-        w.run(warmup, 0)
+        w.run(warmup, warmup)
     segs = hash_segments(steps)
     for nt, he in segs:
         w.prepare_run(nt, he)
@@ -269,6 +274,47 @@ def bench_pillars2048(F, ticks):
     if not out["dense"]["hash"] == out["skip"]["hash"] == out["fused"]["hash"]:
         raise SystemExit("pillars2048: skipping, dense and fused ticks disagree")
     return dict(vox=n ** 3, ticks=ticks, **out)
+
+
+def bench_slab2048(F, ticks=20, slabs=8, halo=2):
+    """One GPU's share of the 2048^3 world cut into `slabs` z-slabs: 2048 x 2048 x
+    (2048 / slabs) planes plus `halo` planes on each side, dense single ticks and
+    fused pairs over all of them, and the device copy of one boundary plane (what
+    the receiving GPU writes of a halo exchange).  The link transfer itself needs
+    a second GPU and is only estimated (one plane each way per link per odd tick
+    at 153 GB/s per xGMI link)."""
+    n, own = 2048, 2048 // slabs
+    nz = own + 2 * halo
+    out = {}
+    for kind in ("dense", "fused"):
+        with F.World((n, n, nz), flags=F.DENSE) as w:
+            w.fuse = kind == "fused"
+            if w.fuse != (kind == "fused"):
+                raise SystemExit("slab2048: fusion not available for the slab shape")
+            w.fill_pillars(2)
+            w.run(4, 0)
+            w.prepare_run(ticks, 0)
+            w.sync()
+            w.run(ticks, 0)
+            out[kind] = w.last_run_ms_per_tick()
+    plane = n * n
+    copy_ms = F.copy_probe(plane, 50)
+    link_ms = plane / XGMI_LINK_BPS * 1e3
+    best = min(out["dense"], out["fused"])
+    tick8 = best + link_ms / 2  # one exchange per two ticks, not overlapped
+    return {"workload": f"one of {slabs} z-slabs of the 2048^3 pillar world: {n}x{n}x{own} planes plus a "
+                        f"{halo}-plane halo on each side ({nz} planes), pillar fill, {ticks} dense ticks",
+            "dims": [n, n, nz], "dense_ms_per_tick": round(out["dense"], 4),
+            "fused_ms_per_tick": round(out["fused"], 4),
+            "own_gvox_per_s": round(plane * own / (best * 1e-3) / 1e9, 1),
+            "halo": {"plane_bytes": plane, "device_copy_ms": round(copy_ms, 4),
+                     "xgmi_ms_per_odd_tick_estimate": round(link_ms, 4),
+                     "note": "per odd tick each GPU sends its two boundary planes to its two z-neighbours over "
+                             "two links at once; the estimate assumes 153 GB/s per link (not measured: one GPU)"},
+            "estimate_8gpu": {"ms_per_tick": round(tick8, 4),
+                              "gvox_per_s": round(n ** 3 / (tick8 * 1e-3) / 1e9, 1),
+                              "note": "the faster of dense and fused slab ticks plus half an unoverlapped "
+                                      "exchange per tick; interior row quads could hide it"}}
 
 
 def orbit_angles(seed, n):
@@ -585,7 +631,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--only", choices=["boxes512", "pillars2048", "copy", "batched", "serving", "render2048",
-                                       "boxes512-window", "dense2048-window"], default=None)
+                                       "slab2048", "boxes512-window", "dense2048-window"], default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs of the timed window")
     args = ap.parse_args()
@@ -628,6 +674,9 @@ def main():
     if args.only == "render2048":
         pr = bench_pillars_render(F)
         print(json.dumps({"workload": "render2048", **pr}), flush=True)
+        return
+    if args.only == "slab2048":
+        print(json.dumps({"workload": "slab2048", **bench_slab2048(F)}), flush=True)
         return
     if args.only == "serving":
         sv = bench_serving(F)
@@ -743,6 +792,7 @@ def main():
             "final_hash": pd["hash"], "segment_bytes": pd["seg"], "dense_kernel": DENSE2048_LABEL,
             "parity": {"tick200_matches_oracle": (pd["hash"] == want200) if want200 is not None else None,
                        "fixture": "tests/golden/pillars2048_frames.json"}}
+        extra["slab2048"] = bench_slab2048(F)
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4),
                          "scatter5_seg32": c.get("scatter5_seg32"), "scatter5_sector64": c.get("scatter5_sector64"),
