@@ -86,8 +86,8 @@ def timed_boxes_window(w, warmup, steps):
     """Warm-up ticks, then the timed graph runs of hash_segments(steps).  Both
     graphs are captured beforehand: the first run has an even tick count, so the
     second starts from the same buffer it was captured on.  The warm-up hashes
-    its last tick, so the hashing kernels' first use in the process (about 50 us
-    of one-time setup) falls outside the timed region.  Returns (wall s, device ms
+    its last tick, so the hashing kernels' first use in the process (tens of
+    microseconds of one-time setup) falls outside the timed region.  Returns (wall s, device ms
     per tick, hashes, absolute tick of each hash)."""
     if warmup:
         w.run(warmup, warmup)
@@ -95,12 +95,15 @@ def timed_boxes_window(w, warmup, steps):
     for nt, he in segs:
         w.prepare_run(nt, he)
     w.sync()
-    hashes, dev = [], 0.0
-    t0 = time.perf_counter()
+    # each run returns when its ticks are done on the device; the wall clock
+    # covers the runs, not the bookkeeping between them
+    hashes, dev, wall = [], 0.0, 0.0
     for nt, he in segs:
-        hashes += [int(x) for x in w.run(nt, he)]
+        t0 = time.perf_counter()
+        out = w.run(nt, he)
+        wall += time.perf_counter() - t0
+        hashes += [int(x) for x in out]
         dev += w.last_run_ms_per_tick() * nt
-    wall = time.perf_counter() - t0
     ticks, t = [], warmup
     for nt, he in segs:
         ticks += [t + he * (i + 1) for i in range(nt // he)]
