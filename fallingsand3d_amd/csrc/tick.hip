@@ -539,7 +539,7 @@ __global__ __launch_bounds__(256) void tick_chunk_kernel(TickArgs a, int chunks)
   if (!HASH && (yq >= nyq || zq >= nzq)) return;  // uniform per wave
   const int seg = chunk * 64 + lane;
   const bool active = seg < LR && yq < nyq && zq < nzq;
-  RowQuad<S> R;
+  RowQuad<S, size_t, true> R;  // the grid streams through: no reuse in cache
   R.seg = active ? seg : 0;
   R.active = active;
   R.yq = active ? yq : 0;

@@ -28,6 +28,11 @@ namespace {
 
 using namespace tickcore;
 
+// The grids these passes serve do not fit in cache: row quads are read from and
+// written to HBM with streaming (nontemporal) accesses (the LDS side ignores it).
+template <int S>
+using StreamRQ = RowQuad<S, size_t, true>;
+
 constexpr int kTY = 3;              // tick-B row quads per workgroup along y
 constexpr int kWaves = kTY + 1;     // waves per workgroup (one tick-A row quad each)
 constexpr int kLayerRows = 2 * kTY; // y rows per LDS layer: the rows tick B reads
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
   const int unused = (wv == 0 ? 0x5 : 0) | (wv == kWaves - 1 ? 0xA : 0);
 
   // tick A's row quad (yqA, s), fetched one step ahead of its update
-  auto fetch_a = [&](RowQuad<S> &R, int seg, int s) {
+  auto fetch_a = [&](StreamRQ<S> &R, int seg, int s) {
     R.seg = seg;
     R.yq = yqA;
     R.zq = s;
@@ -87,7 +92,7 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
     R.skip = false;
     rq_fetch<S>(R, a.src, a.nx, a.ny, a.nz, pA, true);
   };
-  RowQuad<S> R;
+  StreamRQ<S> R;
   fetch_a(R, lane, s_first);
   for (int s = s_first; s <= s_last; s++) {
     const int i = s - s_first;
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
     u8 *const prev_top = lds + (size_t)((2 * i + 2) % 3) * layer_bytes;  // top layer of A's z-row s - 1
     u8 *const cur_bot = lds + (size_t)((2 * i) % 3) * layer_bytes;       // bottom layer of A's z-row s
     u8 *const cur_top = lds + (size_t)((2 * i + 1) % 3) * layer_bytes;   // top layer of A's z-row s
-    RowQuad<S> Rn;
+    StreamRQ<S> Rn;
     fetch_a(Rn, seg, s + 1);
     // ---- tick A: row quad (yqA, s) into LDS ----
     R.seg = seg;
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(kWaves * 64) void tick2_kernel(Tick2Args a) {
     // ---- tick B: row quad (Y0 + wv, s - pA) from LDS into HBM ----
     const int b = s - pA;
     if (wv < kTY && b >= Z0 && b < Z1) {  // uniform per wave; no barrier inside
-      RowQuad<S> Q;
+      StreamRQ<S> Q;
       Q.seg = seg;
       Q.yq = Y0 + wv;
       Q.zq = b;
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
 
   // tick A's row quad (yqA, s), chunk c, fetched one step ahead of its update;
   // the chunk ends' edge bytes come from HBM on odd ticks
-  auto fetch_a = [&](RowQuad<S> &R, int seg, int s) {
+  auto fetch_a = [&](StreamRQ<S> &R, int seg, int s) {
     R.seg = seg;
     R.yq = yqA;
     R.zq = s;
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
       }
     }
   };
-  RowQuad<S> R;
+  StreamRQ<S> R;
   fetch_a(R, c * 64 + lane, s_first);
   for (int s = s_first; s <= s_last; s++) {
     const int i = s - s_first;
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
     u8 *const prev_top = lds + (size_t)((2 * i + 2) % 3) * layer_bytes;  // top layer of A's z-row s - 1
     u8 *const cur_bot = lds + (size_t)((2 * i) % 3) * layer_bytes;       // bottom layer of A's z-row s
     u8 *const cur_top = lds + (size_t)((2 * i + 1) % 3) * layer_bytes;   // top layer of A's z-row s
-    RowQuad<S> Rn;
+    StreamRQ<S> Rn;
     fetch_a(Rn, seg, s + 1);
     // ---- tick A: row quad (yqA, s), chunk c, into LDS ----
     R.valid &= ~unused;
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(1024) void tick2c_kernel(Tick2Args a, int chunks) {
     // ---- tick B: row quad (Y0 + wv, s - pA), chunk c, from LDS into HBM ----
     const int b = s - pA;
     if (wv < kTYc && b >= Z0 && b < Z1) {  // uniform per wave; no barrier inside
-      RowQuad<S> Q;
+      StreamRQ<S> Q;
       Q.seg = seg;
       Q.yq = Y0 + wv;
       Q.zq = b;
