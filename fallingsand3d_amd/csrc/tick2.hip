@@ -29,9 +29,11 @@ namespace {
 using namespace tickcore;
 
 // The grids these passes serve do not fit in cache: row quads are read from and
-// written to HBM with streaming (nontemporal) accesses (the LDS side ignores it).
+// written to HBM with streaming (nontemporal) accesses; the LDS side ignores the
+// hint (2048^3 pillar scene, ms per fused tick: 2.36 streaming loads and stores,
+// 2.40 streaming stores only, 2.39 neither; 1024^3 boxes: 0.313 / 0.315 / 0.323).
 template <int S>
-using StreamRQ = RowQuad<S, size_t, true>;
+using StreamRQ = RowQuad<S, size_t, kStreamLoads | kStreamStores>;
 
 constexpr int kTY = 3;              // tick-B row quads per workgroup along y
 constexpr int kWaves = kTY + 1;     // waves per workgroup (one tick-A row quad each)

@@ -169,9 +169,10 @@ struct SkipCtx {
 // 2*dz), loaded, plus where it lives.  Out-of-grid rows read as STATIC.
 // Off is the type of the rows' byte offsets: u32 for grids of at most 2^32
 // bytes (one register per address, a 32-bit offset from the uniform base
-// pointer), else size_t.  NT: the segment is loaded and stored with streaming
-// (nontemporal) accesses.
-template <int S, class Off = size_t, bool NT = false>
+// pointer), else size_t.  ST: which of the segment's loads (kStreamLoads) and
+// stores (kStreamStores) are streaming (nontemporal) accesses.
+constexpr int kStreamLoads = 1, kStreamStores = 2;
+template <int S, class Off = size_t, int ST = 0>
 struct RowQuad {
   u32 own[4][S / 4];
   Off rowoff[4];
@@ -193,8 +194,8 @@ struct RowQuad {
 
 // Loads the lane's segment of the 4 rows of row quad (R.yq, R.zq) at parity p
 // (R.seg and R.active set); fetch = false leaves placeholders (STATIC).
-template <int S, class Off, bool NT>
-__device__ __forceinline__ void rq_fetch(RowQuad<S, Off, NT> &R, const u8 *src, int nx, int ny, int nz, int p,
+template <int S, class Off, int ST>
+__device__ __forceinline__ void rq_fetch(RowQuad<S, Off, ST> &R, const u8 *src, int nx, int ny, int nz, int p,
                                          bool fetch) {
   constexpr int ND = S / 4;
   R.loaded = fetch;
@@ -210,7 +211,7 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S, Off, NT> &R, const u8 *src, 
     R.rowoff[r] = (Off)(((size_t)(ok ? z : 0) * ny + (ok ? y : 0)) * nx + xs);
 #pragma unroll
     for (int j = 0; j < ND; j++) R.own[r][j] = kStatic4;
-    if (ok && fetch) load_seg<S, NT>(src + R.rowoff[r], R.own[r]);
+    if (ok && fetch) load_seg<S, (ST & kStreamLoads) != 0>(src + R.rowoff[r], R.own[r]);
   }
   // a partial last segment: bytes past the row read as STATIC (after all four
   // loads are issued, so that they overlap)
@@ -232,8 +233,8 @@ __device__ __forceinline__ void rq_fetch(RowQuad<S, Off, NT> &R, const u8 *src, 
 // edge words, which follow in the same allocation (512^3 box scene: 45.3
 // against 46.1 us per tick over ticks 5-25 with the last hashed, 35.8 against
 // 36.7 over ticks 100-1100, with five 8-byte loads).
-template <int S, class Off, bool NT>
-__device__ __forceinline__ void rq_skip_check(RowQuad<S, Off, NT> &R, int p, const SkipCtx *sk) {
+template <int S, class Off, int ST>
+__device__ __forceinline__ void rq_skip_check(RowQuad<S, Off, ST> &R, int p, const SkipCtx *sk) {
   u64x2 v[2];
   bool in[4];  // word (y = yq + (k & 1) - p, z = zq + (k >> 1) - p) is a row quad
   long long i0[2];
@@ -301,8 +302,8 @@ __device__ __forceinline__ void rq_load(RowQuad<S> &R, const u8 *src, int nx, in
 // left / right neighbour's cell (R.sl / R.sr) because it changed, and bit 2 /
 // bit 3 when the lane's first / last block changed (on odd ticks the blocks
 // straddling its left / right edge).
-template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P, class Off, bool NT>
-__device__ __forceinline__ u64 rq_update_p(const RowQuad<S, Off, NT> &R, u8 *dst, int LR, u32 t,
+template <int S, bool HASH, bool SKIP, bool TRACK, bool EDGES, int P, class Off, int ST>
+__device__ __forceinline__ u64 rq_update_p(const RowQuad<S, Off, ST> &R, u8 *dst, int LR, u32 t,
                                            const u64 *__restrict__ rule, bool *changed, u32 *edge_changed) {
   constexpr int ND = S / 4;  // dwords per lane per row
   const int p = P < 0 ? (int)(t & 1u) : P;
@@ -493,7 +494,7 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S, Off, NT> &R, u8 *dst
       if (SKIP && R.skip) {
         // settled: the destination already holds these bytes
       } else if (R.m == S) {
-        store_seg<S, NT>(dp, res);
+        store_seg<S, (ST & kStreamStores) != 0>(dp, res);
       } else {
 #pragma unroll
         for (int k = 0; k < S; k++)
@@ -524,8 +525,8 @@ __device__ __forceinline__ u64 rq_update_p(const RowQuad<S, Off, NT> &R, u8 *dst
 // tick's parity is the same across a wave in every caller (kernel arguments, or
 // one world per workgroup).
 template <int S, bool HASH, bool SKIP = false, bool TRACK = SKIP, bool EDGES = false, class Off = size_t,
-          bool NT = false>
-__device__ __forceinline__ u64 rq_update(const RowQuad<S, Off, NT> &R, u8 *dst, int LR, u32 t,
+          int ST = 0>
+__device__ __forceinline__ u64 rq_update(const RowQuad<S, Off, ST> &R, u8 *dst, int LR, u32 t,
                                          const u64 *__restrict__ rule, bool *changed = nullptr,
                                          u32 *edge_changed = nullptr) {
   if (t & 1u) return rq_update_p<S, HASH, SKIP, TRACK, EDGES, 1>(R, dst, LR, t, rule, changed, edge_changed);
