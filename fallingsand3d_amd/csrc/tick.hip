@@ -423,7 +423,10 @@ __global__ __launch_bounds__(256) void tick_pack_kernel(TickArgs a, const TickEn
       // neighbour segments being updated in this group or another one
       const bool left_act = lv && seg > 0 && (h > 0 || ((ao >> (sj - 1)) & 1ull));
       const bool right_act = lv && seg < LR - 1 && (h < G - 1 || ((ao >> (sj + 1)) & 1ull));
-      RowQuad<S> R;
+      // streaming stores: the active sectors of a large grid leave cache before the
+      // next tick reads them (2048^3 pillar scene: 0.786 against 0.806 ms per tick;
+      // streaming loads as well: 0.827)
+      RowQuad<S, size_t, kStreamStores> R;
       R.seg = seg;
       R.active = lv;
       R.yq = (int)yq;
