@@ -21,8 +21,9 @@ Default run (one JSON line on stdout):
            fit the 256 MB Infinity Cache: the HBM-bound figure.
   extra    pillars2048: the 2048^3 pillar scene, skipping, dense and dense fused
            in pairs; pillars2048_render: that scene with a 1920x1080 voxel render
-           every 10 ticks; copy: a plain device copy of the same bytes, the
-           practical bandwidth ceiling; batched: 4096 worlds against the
+           every 10 ticks; copy: a plain device copy of the same bytes, with
+           temporal and with streaming (nontemporal) accesses, the practical
+           bandwidth ceilings; batched: 4096 worlds against the
            one-launch-per-world loop; serving: the continuous-batching server
            under a Poisson load, saturated, and against one world per request;
            slab2048: one GPU's z-slab of the 2048^3 world (dense and fused ticks
@@ -427,13 +428,18 @@ def voxel_steps(cam, dims, hit, W, H):
 
 
 def bench_copy(F, nbytes, iters=20, scatter=True):
-    """The streaming copy, and the packed skipping tick's access shape without
-    its arithmetic: 5 scattered 32-byte segments of every row quad of a 2048^3
-    grid pair (about the tick's density on the pillar scene), copied as 32-byte
-    pieces and as whole 64-byte sectors."""
+    """The plain copy (temporal and streaming loads and stores), and the packed
+    skipping tick's access shape without its arithmetic: 5 scattered 32-byte
+    segments of every row quad of a 2048^3 grid pair (about the tick's density
+    on the pillar scene), copied as 32-byte pieces and as whole 64-byte
+    sectors."""
     ms = F.copy_probe(nbytes, iters)
     log(f"[bench] copy: {nbytes} bytes in {ms:.4f} ms ({2 * nbytes / ms / 1e6:.1f} GB/s)")
     out = dict(nbytes=nbytes, ms=ms, GBps=2 * nbytes / (ms * 1e-3) / 1e9)
+    # with streaming (nontemporal) loads and stores, as the large-grid passes use
+    ms_nt = F.copy_probe(nbytes, iters, streaming=True)
+    out["streaming"] = dict(ms=round(ms_nt, 4), GBps=round(2 * nbytes / (ms_nt * 1e-3) / 1e9, 1))
+    log(f"[bench] streaming copy: {2 * nbytes / ms_nt / 1e6:.1f} GB/s")
     # the same copy with a 128 MiB working set (64 MiB read, 64 MiB written), which
     # stays in the 256 MB Infinity Cache like an in-place 512^3 tick's grid
     small = 64 << 20
@@ -441,10 +447,8 @@ def bench_copy(F, nbytes, iters=20, scatter=True):
     out["cache_resident"] = dict(nbytes=small, ms=round(ms_s, 5), GBps=round(2 * small / (ms_s * 1e-3) / 1e9, 1))
     log(f"[bench] copy, 128 MiB working set: {2 * small / ms_s / 1e6:.1f} GB/s")
     if scatter:
-        # 5 of the 64 segments: the packed tick's shape; all 64: the dense tick's
-        # (every row quad's four rows, one 32-byte segment of each per lane)
-        for key, per_rq, sector in (("scatter5_seg32", 5, False), ("scatter5_sector64", 5, True),
-                                    ("rowquads_all", 64, False)):
+        # 5 of the 64 segments: about the packed tick's density on the pillar scene
+        for key, per_rq, sector in (("scatter5_seg32", 5, False), ("scatter5_sector64", 5, True)):
             sms, sb = F.scatter_probe(per_rq, sector)
             out[key] = dict(ms=round(sms, 4), item_bytes=sb, GBps=round(sb / (sms * 1e-3) / 1e9, 1))
             log(f"[bench] {key}: {sb / 1e9:.3f} GB of items in {sms:.4f} ms")
@@ -751,10 +755,8 @@ def main():
                "frac": round(frac2, 4), "traffic": None,
                "kernel": DENSE2048_LABEL,
                "kernel_ms": round(p["dense"]["kern_ms"], 5), "alg_bytes_per_tick": 2 * p["vox"],
-               "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4)}
-        if c.get("rowquads_all"):
-            # against the same access shape without arithmetic (every row quad's four rows)
-            hbm["frac_of_access_shape"] = round(ach2 / 1e9 / c["rowquads_all"]["GBps"], 4)
+               "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
+               "frac_of_streaming_copy": round(ach2 / 1e9 / c["streaming"]["GBps"], 4)}
         if traffic2048 is not None:
             if "error" in traffic2048:
                 hbm["traffic_error"] = traffic2048["error"]
@@ -786,6 +788,7 @@ def main():
             "dense_gvox_per_s": round(p["vox"] * p["ticks"] / pd["wall"] / 1e9, 2),
             "dense_kernel_ms": round(pd["kern_ms"], 4), "dense_achieved_GBps": round(ach2 / 1e9, 1),
             "dense_hbm_frac": round(frac2, 4), "dense_frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
+            "dense_frac_of_streaming_copy": round(ach2 / 1e9 / c["streaming"]["GBps"], 4),
             "fused_pairs": {"kernel": "tick2c_kernel<16> (two dense ticks per pass in 16-byte x-chunks of 1 KiB, "
                                              "first tick's rows in LDS)",
                             "gvox_per_s": round(p["vox"] * p["ticks"] / p["fused"]["wall"] / 1e9, 2),
@@ -797,9 +800,9 @@ def main():
                        "fixture": "tests/golden/pillars2048_frames.json"}}
         extra["slab2048"] = bench_slab2048(F)
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
-                         "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4),
+                         "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4), "streaming": c["streaming"],
                          "scatter5_seg32": c.get("scatter5_seg32"), "scatter5_sector64": c.get("scatter5_sector64"),
-                         "cache_resident": c.get("cache_resident"), "rowquads_all": c.get("rowquads_all")}
+                         "cache_resident": c.get("cache_resident")}
         bb = bench_batched(F)
         extra["batched"] = {
             "workload": "4096 worlds, edges log-uniform 16-128, rain fill, density U(0.05,0.5), 500 ticks",

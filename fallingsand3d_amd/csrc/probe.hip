@@ -16,10 +16,16 @@ namespace {
 
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 
+template <bool NT>
 __global__ __launch_bounds__(256) void copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
                                                    size_t n16) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n16) dst[i] = src[i];
+  if (i < n16) {
+    if constexpr (NT)
+      __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    else
+      dst[i] = src[i];
+  }
 }
 
 __device__ __forceinline__ u32 probe_mix(u32 h) {
@@ -69,7 +75,9 @@ __global__ __launch_bounds__(256) void scatter_kernel(const u8 *__restrict__ src
 
 using namespace fs3d;
 
-extern "C" int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy) {
+namespace {
+
+int copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy, bool streaming) {
   if (!ms_per_copy || iters < 1 || nbytes < 16 || (nbytes & 15))
     return set_error(FS3D_ERR_INVALID, "copy probe needs nbytes >= 16, a multiple of 16, and iters >= 1");
   const size_t n16 = nbytes / 16;
@@ -88,15 +96,16 @@ extern "C" int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) e = hipMemsetAsync(a, 1, nbytes, s);
   const unsigned blocks = (unsigned)nblocks;
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n16);
-    e = hipGetLastError();
-  }
+  auto launch = [&]() {
+    if (streaming)
+      hipLaunchKernelGGL(copy_kernel<true>, dim3(blocks), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n16);
+    else
+      hipLaunchKernelGGL(copy_kernel<false>, dim3(blocks), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n16);
+    return hipGetLastError();
+  };
+  if (e == hipSuccess) e = launch();  // warm-up
   if (e == hipSuccess) e = hipEventRecord(e0, s);
-  for (int i = 0; i < iters && e == hipSuccess; i++) {
-    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n16);
-    e = hipGetLastError();
-  }
+  for (int i = 0; i < iters && e == hipSuccess; i++) e = launch();
   if (e == hipSuccess) e = hipEventRecord(e1, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   float ms = 0.f;
@@ -110,6 +119,16 @@ extern "C" int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy
   if (first != hipSuccess) return set_error(FS3D_ERR_HIP, "copy probe failed: %s", hipGetErrorString(first));
   *ms_per_copy = (double)ms / iters;
   return clear_error();
+}
+
+}  // namespace
+
+extern "C" int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy) {
+  return copy_probe(nbytes, iters, ms_per_copy, false);
+}
+
+extern "C" int fs3d_copy_probe_streaming(size_t nbytes, int32_t iters, double *ms_per_copy) {
+  return copy_probe(nbytes, iters, ms_per_copy, true);
 }
 
 extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass,

@@ -43,10 +43,12 @@ def rule_lut():
     return lut
 
 
-def copy_probe(nbytes, iters=20):
-    """Mean ms of a plain device copy of nbytes (bandwidth calibration)."""
+def copy_probe(nbytes, iters=20, streaming=False):
+    """Mean ms of a plain device copy of nbytes (bandwidth calibration); with
+    streaming, the copy uses nontemporal loads and stores."""
     ms = c_double(0)
-    check(lib().fs3d_copy_probe(nbytes, iters, ctypes.byref(ms)))
+    fn = lib().fs3d_copy_probe_streaming if streaming else lib().fs3d_copy_probe
+    check(fn(nbytes, iters, ctypes.byref(ms)))
     return ms.value
 
 

@@ -425,6 +425,11 @@ int fs3d_world_last_run_ms(const fs3d_world *w, double *ms_per_tick);
  * compared with, and the known-size kernel that scales rocprof byte counters.
  * No reference counterpart (the reference has no timing at all). */
 int fs3d_copy_probe(size_t nbytes, int32_t iters, double *ms_per_copy);
+/* The same copy with streaming (nontemporal) loads and stores: the ceiling for
+ * the passes that stream their rows (the chunked dense tick's stores, the fused
+ * passes' loads and stores, the packed skipping tick's stores).  No reference
+ * counterpart. */
+int fs3d_copy_probe_streaming(size_t nbytes, int32_t iters, double *ms_per_copy);
 /* Mean device time (ms) of one pass of the packed skipping tick's access
  * shape without its arithmetic, on a 2048^3 grid pair (16 GiB, allocated and
  * freed by the call): per_rq (1..64) scattered, distinct 32-byte segments
