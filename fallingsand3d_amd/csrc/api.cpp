@@ -210,9 +210,12 @@ void plan_fused(fs3d_world *w) {
 
 // z row-quads per workgroup of the fused two-tick pass.  Eight keep enough
 // workgroups in flight on smaller grids (us per fused tick, z = 4 / 8 / 16 /
-// 32: 512^3 47.3 / 48.4 / 49.2 / 69.5; 1024^3 336 / 324 / 326 / 341; 2048^3
-// 2530 / 2400 / 2400 / 2400); the z halo costs 1/8 more tick-A rows.
-constexpr int kTick2ZLen = 8;
+// 32: 512^3 47.3 / 48.4 / 49.2 / 69.5; 1024^3 336 / 324 / 326 / 341); the z
+// halo costs 1/8 more tick-A rows.  Above 2^32 voxels there are workgroups
+// enough for sixteen, whose halo costs half as much (2048^3 with streaming
+// accesses: 2.30-2.31 against 2.36 ms per tick at eight and 2.33 at 32; 1024^3
+// at sixteen: 0.319 against 0.312).
+int tick2_zlen(const fs3d_world *w) { return (u64)w->nx * (u64)w->ny * (u64)w->nz > (1ull << 32) ? 16 : 8; }
 
 // Dense single ticks of rows longer than 1024 bytes (with the automatic
 // segment) run one wave per 16-byte x-chunk of 1 KiB instead of one wave per
@@ -252,7 +255,7 @@ int enqueue_ticks(fs3d_world *w, int nticks, int hash_every, const u64 *tick_dev
     if (fuse && k + 1 < nticks && !hashed_after(k) && !hashed_after(k + 1)) {
       if (launch)
         FS3D_HIP(launch_tick2(w->shape2, w->buf[src], w->buf[src ^ 1], w->nx, w->ny, w->nz, w->tick + (u64)k,
-                              tick_dev, (u32)k, rule, kTick2ZLen, w->stream));
+                              tick_dev, (u32)k, rule, tick2_zlen(w), w->stream));
       src ^= 1;
       k += 2;
       continue;
