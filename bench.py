@@ -428,11 +428,12 @@ def voxel_steps(cam, dims, hit, W, H):
 
 
 def bench_copy(F, nbytes, iters=20, scatter=True):
-    """The plain copy (temporal and streaming loads and stores), and the packed
-    skipping tick's access shape without its arithmetic: 5 scattered 32-byte
-    segments of every row quad of a 2048^3 grid pair (about the tick's density
-    on the pillar scene), copied as 32-byte pieces and as whole 64-byte
-    sectors."""
+    """The plain copy (temporal and streaming loads and stores); the dense
+    chunked tick's access shape without its arithmetic (every row quad's four
+    rows of a 2048^3 grid pair, one wave per 1 KiB x-chunk); and the packed
+    skipping tick's: 5 scattered 32-byte segments of every row quad (about the
+    tick's density on the pillar scene), copied as 32-byte pieces and as whole
+    64-byte sectors."""
     ms = F.copy_probe(nbytes, iters)
     log(f"[bench] copy: {nbytes} bytes in {ms:.4f} ms ({2 * nbytes / ms / 1e6:.1f} GB/s)")
     out = dict(nbytes=nbytes, ms=ms, GBps=2 * nbytes / (ms * 1e-3) / 1e9)
@@ -447,6 +448,12 @@ def bench_copy(F, nbytes, iters=20, scatter=True):
     out["cache_resident"] = dict(nbytes=small, ms=round(ms_s, 5), GBps=round(2 * small / (ms_s * 1e-3) / 1e9, 1))
     log(f"[bench] copy, 128 MiB working set: {2 * small / ms_s / 1e6:.1f} GB/s")
     if scatter:
+        # the dense chunked tick's loads and stores without its arithmetic, with
+        # the tick's own policy (temporal loads, streaming stores)
+        rms = F.rowquad_probe(False, True)
+        out["rowquad_chunks"] = dict(ms=round(rms, 4), GBps=round(2 * 2048 ** 3 / (rms * 1e-3) / 1e9, 1),
+                                     policy="temporal loads, streaming stores")
+        log(f"[bench] rowquad_chunks: {rms:.4f} ms per pass of a 2048^3 pair")
         # 5 of the 64 segments: about the packed tick's density on the pillar scene
         for key, per_rq, sector in (("scatter5_seg32", 5, False), ("scatter5_sector64", 5, True)):
             sms, sb = F.scatter_probe(per_rq, sector)
@@ -757,6 +764,9 @@ def main():
                "kernel_ms": round(p["dense"]["kern_ms"], 5), "alg_bytes_per_tick": 2 * p["vox"],
                "frac_of_copy": round(ach2 / 1e9 / c["GBps"], 4),
                "frac_of_streaming_copy": round(ach2 / 1e9 / c["streaming"]["GBps"], 4)}
+        if c.get("rowquad_chunks"):
+            # against its own loads and stores without the arithmetic
+            hbm["frac_of_access_shape"] = round(ach2 / 1e9 / c["rowquad_chunks"]["GBps"], 4)
         if traffic2048 is not None:
             if "error" in traffic2048:
                 hbm["traffic_error"] = traffic2048["error"]
@@ -802,7 +812,7 @@ def main():
         extra["copy"] = {"bytes": c["nbytes"], "ms": round(c["ms"], 4), "GBps": round(c["GBps"], 1),
                          "hbm_frac": round(c["GBps"] * 1e9 / HBM_PEAK, 4), "streaming": c["streaming"],
                          "scatter5_seg32": c.get("scatter5_seg32"), "scatter5_sector64": c.get("scatter5_sector64"),
-                         "cache_resident": c.get("cache_resident")}
+                         "cache_resident": c.get("cache_resident"), "rowquad_chunks": c.get("rowquad_chunks")}
         bb = bench_batched(F)
         extra["batched"] = {
             "workload": "4096 worlds, edges log-uniform 16-128, rain fill, density U(0.05,0.5), 500 ticks",

@@ -104,6 +104,7 @@ SIGNATURES = {
     "fs3d_copy_probe": (c_int, [c_size, c_i32, P(c_double)]),
     "fs3d_copy_probe_streaming": (c_int, [c_size, c_i32, P(c_double)]),
     "fs3d_scatter_probe": (c_int, [c_i32, c_i32, c_i32, P(c_double), P(c_double)]),
+    "fs3d_rowquad_probe": (c_int, [c_i32, c_i32, c_i32, P(c_double)]),
     "fs3d_batch_create": (c_int, [c_i32, P(c_i32), P(vp)]),
     "fs3d_batch_destroy": (c_int, [vp]),
     "fs3d_batch_size": (c_int, [vp, P(c_i32)]),

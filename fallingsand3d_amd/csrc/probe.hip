@@ -70,6 +70,36 @@ __global__ __launch_bounds__(256) void scatter_kernel(const u8 *__restrict__ src
     for (int c = 0; c < kV; c++) *(u32x4 *)(dst + off[r] + x + 16 * c) = v[r][c];
 }
 
+// The dense chunked tick's access shape without its arithmetic, on an n^3 grid
+// pair (even parity): wave w copies x-chunk w % 2 (1 KiB: 16 bytes per lane) of
+// the four rows of row quad w / 2, with streaming (nontemporal) loads (NTL)
+// and stores (NTS) or temporal ones.  n = 2048: two chunks per row.
+template <bool NTL, bool NTS>
+__global__ __launch_bounds__(256) void rowquad_chunk_kernel(const u8 *__restrict__ src, u8 *__restrict__ dst,
+                                                            int n) {
+  const int lane = threadIdx.x & 63;
+  const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const size_t chunk = w & 1, rq = w >> 1;
+  const size_t nrq_y = (size_t)n / 2;
+  const size_t yq = rq % nrq_y, zq = rq / nrq_y;
+  const size_t base = ((2 * zq) * (size_t)n + 2 * yq) * (size_t)n + chunk * 1024 + (size_t)lane * 16;
+  const size_t off[4] = {base, base + n, base + (size_t)n * n, base + (size_t)n * n + n};
+  u32x4 v[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const u32x4 *p = (const u32x4 *)(src + off[r]);
+    v[r] = NTL ? __builtin_nontemporal_load(p) : *p;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    u32x4 *p = (u32x4 *)(dst + off[r]);
+    if constexpr (NTS)
+      __builtin_nontemporal_store(v[r], p);
+    else
+      *p = v[r];
+  }
+}
+
 }  // namespace
 }  // namespace fs3d
 
@@ -131,14 +161,13 @@ extern "C" int fs3d_copy_probe_streaming(size_t nbytes, int32_t iters, double *m
   return copy_probe(nbytes, iters, ms_per_copy, true);
 }
 
-extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass,
-                                  double *item_bytes) {
-  constexpr int n = 2048;
-  if (!ms_per_pass || !item_bytes || iters < 1 || per_rq < 1 || per_rq > 64)
-    return set_error(FS3D_ERR_INVALID, "scatter probe needs 1 <= per_rq <= 64 and iters >= 1");
-  const size_t nbytes = (size_t)n * n * n;
-  const size_t nitems = (size_t)(n / 2) * (n / 2) * (size_t)per_rq;
-  const unsigned blocks = (unsigned)((nitems + 255) / 256);
+namespace {
+
+// Mean ms per launch of `launch(src, dst, stream)` over a 2048^3 grid pair
+// (16 GiB, allocated and freed here), after one warm-up launch.
+template <class F>
+int time_on_pair(int32_t iters, const char *what, F launch, double *ms_per_pass) {
+  constexpr size_t nbytes = (size_t)2048 * 2048 * 2048;
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (int rc = ensure_device(dev)) return rc;
@@ -154,12 +183,7 @@ extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters,
   if (e == hipSuccess) e = hipMemsetAsync(b, 0, nbytes, s);
   for (int i = 0; i <= iters && e == hipSuccess; i++) {  // launch 0 warms up
     if (i == 1) e = hipEventRecord(e0, s);
-    if (e != hipSuccess) break;
-    if (sector)
-      hipLaunchKernelGGL(scatter_kernel<true>, dim3(blocks), dim3(256), 0, s, (const u8 *)a, (u8 *)b, n, nitems, per_rq);
-    else
-      hipLaunchKernelGGL(scatter_kernel<false>, dim3(blocks), dim3(256), 0, s, (const u8 *)a, (u8 *)b, n, nitems, per_rq);
-    e = hipGetLastError();
+    if (e == hipSuccess) e = launch((const u8 *)a, (u8 *)b, s);
   }
   if (e == hipSuccess) e = hipEventRecord(e1, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -171,8 +195,47 @@ extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters,
   if (s) (void)hipStreamDestroy(s);
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
-  if (first != hipSuccess) return set_error(FS3D_ERR_HIP, "scatter probe failed: %s", hipGetErrorString(first));
+  if (first != hipSuccess) return set_error(FS3D_ERR_HIP, "%s failed: %s", what, hipGetErrorString(first));
   *ms_per_pass = (double)ms / iters;
+  return clear_error();
+}
+
+}  // namespace
+
+extern "C" int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass,
+                                  double *item_bytes) {
+  constexpr int n = 2048;
+  if (!ms_per_pass || !item_bytes || iters < 1 || per_rq < 1 || per_rq > 64)
+    return set_error(FS3D_ERR_INVALID, "scatter probe needs 1 <= per_rq <= 64 and iters >= 1");
+  const size_t nitems = (size_t)(n / 2) * (n / 2) * (size_t)per_rq;
+  const unsigned blocks = (unsigned)((nitems + 255) / 256);
+  auto launch = [&](const u8 *a, u8 *b, hipStream_t s) {
+    if (sector)
+      hipLaunchKernelGGL(scatter_kernel<true>, dim3(blocks), dim3(256), 0, s, a, b, n, nitems, per_rq);
+    else
+      hipLaunchKernelGGL(scatter_kernel<false>, dim3(blocks), dim3(256), 0, s, a, b, n, nitems, per_rq);
+    return hipGetLastError();
+  };
+  if (int rc = time_on_pair(iters, "scatter probe", launch, ms_per_pass)) return rc;
   *item_bytes = (double)nitems * 4 * (sector ? 64 : 32) * 2;
   return clear_error();
+}
+
+extern "C" int fs3d_rowquad_probe(int32_t streaming_loads, int32_t streaming_stores, int32_t iters,
+                                  double *ms_per_pass) {
+  constexpr int n = 2048;
+  if (!ms_per_pass || iters < 1) return set_error(FS3D_ERR_INVALID, "row-quad probe needs iters >= 1");
+  const unsigned blocks = (unsigned)((size_t)(n / 2) * (n / 2) * 2 / 4);  // two chunk waves per row quad
+  auto launch = [&](const u8 *a, u8 *b, hipStream_t s) {
+    if (streaming_loads && streaming_stores)
+      hipLaunchKernelGGL((rowquad_chunk_kernel<true, true>), dim3(blocks), dim3(256), 0, s, a, b, n);
+    else if (streaming_loads)
+      hipLaunchKernelGGL((rowquad_chunk_kernel<true, false>), dim3(blocks), dim3(256), 0, s, a, b, n);
+    else if (streaming_stores)
+      hipLaunchKernelGGL((rowquad_chunk_kernel<false, true>), dim3(blocks), dim3(256), 0, s, a, b, n);
+    else
+      hipLaunchKernelGGL((rowquad_chunk_kernel<false, false>), dim3(blocks), dim3(256), 0, s, a, b, n);
+    return hipGetLastError();
+  };
+  return time_on_pair(iters, "row-quad probe", launch, ms_per_pass);
 }

@@ -52,6 +52,16 @@ def copy_probe(nbytes, iters=20, streaming=False):
     return ms.value
 
 
+def rowquad_probe(streaming_loads, streaming_stores, iters=10):
+    """ms per pass of the dense chunked tick's access shape (every row quad's
+    four rows, one wave per 1 KiB x-chunk) on a 2048^3 grid pair, with streaming
+    or temporal loads and stores."""
+    ms = c_double(0)
+    check(lib().fs3d_rowquad_probe(1 if streaming_loads else 0, 1 if streaming_stores else 0, iters,
+                                   ctypes.byref(ms)))
+    return ms.value
+
+
 def scatter_probe(per_rq, sector, iters=10):
     """(ms per pass, item bytes) of the packed tick's access shape on a 2048^3
     grid pair: per_rq scattered 32-byte segments of every row quad (sector:

@@ -438,6 +438,13 @@ int fs3d_copy_probe_streaming(size_t nbytes, int32_t iters, double *ms_per_copy)
  * instead.  *item_bytes receives the bytes the items read and write.  No
  * reference counterpart. */
 int fs3d_scatter_probe(int32_t per_rq, int32_t sector, int32_t iters, double *ms_per_pass, double *item_bytes);
+/* Mean device time (ms) of one pass of the dense chunked tick's access shape
+ * without its arithmetic, on a 2048^3 grid pair (16 GiB, allocated and freed
+ * by the call): every row quad's four rows copied from one grid to the other,
+ * one wave per 1 KiB x-chunk, 16 bytes per lane and row; streaming_loads /
+ * streaming_stores != 0 make those accesses streaming (nontemporal).  No
+ * reference counterpart. */
+int fs3d_rowquad_probe(int32_t streaming_loads, int32_t streaming_stores, int32_t iters, double *ms_per_pass);
 
 #ifdef __cplusplus
 }

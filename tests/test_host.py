@@ -93,6 +93,9 @@ def test_probes_reject_bad_arguments_before_device():
         with pytest.raises(F.Fs3dError) as ei:
             F.copy_probe(15, streaming=streaming)
         assert ei.value.code == 1
+    with pytest.raises(F.Fs3dError) as ei:
+        F.rowquad_probe(False, True, 0)
+    assert ei.value.code == 1
 
 
 def test_last_error_is_thread_local_message():
